@@ -1,0 +1,463 @@
+// Mass assignment with interlacing (powerI4.f90:68-249) on gfx950.
+//
+// Particles are first binned by tile (T^3 cells, T <= 16) with a counting sort, then one
+// work-group per tile deposits its particles into LDS copies of the tile plus a 2-cell halo on
+// each side ((T+4)^3 cells per grid), accumulating in 64-bit fixed point (2^-32 units) so the
+// result does not depend on the order of the LDS atomics.  Each tile writes its halo-extended
+// block to a region buffer with plain coalesced stores; a gather pass then sums, for every
+// grid cell, the (1 to 8) region entries that cover it in a fixed order.  No global atomics
+// touch grid memory, and the output is bitwise reproducible.
+//
+// Stencils relative to the cell c = floor(u) containing the particle (u = N * pos), written
+// with the fractional part f = u - c, follow the reference exactly:
+//   PCS  grid 1: cells c-1..c+2, h = f                         (powerI4.f90:164-186)
+//        grid 2: position u - 1/2                              (vt = vr - 0.5, :162)
+//   TSC  grid 1: cells nint(u)-1..+1, h = u - nint(u)          (:187-207)
+//   CIC  grid 1: cells c+1, c+2, h = f  (a uniform +1 shift)   (:208-224)
+//        single grid: h = u - nint(u)                          (:114-124)
+//   NGP  cell nint(u)                                          (:225-233)
+#include "common.h"
+
+namespace pbi4 {
+
+namespace {
+
+constexpr int kHalo = 2;          // region cells below the tile origin
+constexpr int kMaxR = 16 + 4;     // T <= 16
+constexpr int kDepositThreads = 1024;
+constexpr int kSortThreads = 1024;
+constexpr int64_t kLdsHistMax = 32768;   // tiles counted in LDS (u32) per work-group
+constexpr float kFix = 4294967296.0f;    // 2^32 fixed-point scale
+constexpr float kInvFix = 2.3283064365386963e-10f;
+
+struct PosXform {
+    int N, pos_mode;
+    float box, rsd[3];
+    int has_vel;
+};
+
+// Grid coordinate of one component: integer cell c in [0, N) and fraction f in [0, 1), with
+// u = N * p evaluated with its FMA residual so f is accurate to ~2^-24 cells for any N.
+__device__ __forceinline__ void grid_coord(const PosXform &x, float p, int &c, float &f) {
+    const float Nf = (float)x.N;
+    float u = Nf * p;
+    float e = fmaf(Nf, p, -u);
+    float fl = floorf(u);
+    f = (u - fl) + e;
+    int ci = (int)fl;
+    if (f < 0.0f) {
+        ci -= 1;
+        f += 1.0f;
+    } else if (f >= 1.0f) {
+        ci += 1;
+        f -= 1.0f;
+    }
+    if (f >= 1.0f) f = 0.99999994f;
+    ci %= x.N;
+    if (ci < 0) ci += x.N;
+    c = ci;
+}
+
+// Normalised position component as the reference computes it: (s + box) % box / box in
+// float32 after the RSD shift (pbi4.py:154, 242-243), or the given value if already in [0,1).
+__device__ __forceinline__ float norm_pos(const PosXform &x, const float *pos, const float *vel,
+                                          int d, int64_t i, int64_t npart) {
+    float p = pos[(int64_t)d * npart + i];
+    if (x.pos_mode == PBI4_POS_NORMALISED) return p;
+    float s = p;
+    if (x.has_vel) s = fmaf(x.rsd[d], vel[(int64_t)d * npart + i], s);
+    float t = fmodf(s + x.box, x.box);
+    if (t < 0.0f) t += x.box;
+    float q = t / x.box;
+    return fminf(q, 1.0f);
+}
+
+__device__ __forceinline__ int tile_of(const PosXform &x, int T, int NT, const float *pos,
+                                       const float *vel, int64_t i, int64_t npart,
+                                       uint32_t *packed) {
+    int c[3];
+    float f[3];
+    for (int d = 0; d < 3; ++d) grid_coord(x, norm_pos(x, pos, vel, d, i, npart), c[d], f[d]);
+    int t[3];
+    for (int d = 0; d < 3; ++d) {
+        t[d] = c[d] / T;
+        if (packed) {
+            uint32_t loc = (uint32_t)(c[d] - t[d] * T);
+            uint32_t fr = (uint32_t)(f[d] * 268435456.0f);   // 2^28
+            if (fr > 0x0FFFFFFFu) fr = 0x0FFFFFFFu;
+            packed[d] = (loc << 28) | fr;
+        }
+    }
+    return (t[2] * NT + t[1]) * NT + t[0];
+}
+
+// ------------------------------------------------------------------ counting sort
+
+__global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, int NT,
+                                                            int64_t ntiles, const float *pos,
+                                                            const float *vel, int64_t npart,
+                                                            uint32_t *counts, int use_lds) {
+    extern __shared__ uint32_t lhist[];
+    const int64_t per_wg = ceil_div(npart, (int64_t)gridDim.x);
+    const int64_t beg = (int64_t)blockIdx.x * per_wg;
+    const int64_t end = min(npart, beg + per_wg);
+    if (use_lds) {
+        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
+        __syncthreads();
+        for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
+            atomicAdd(&lhist[tile_of(x, T, NT, pos, vel, i, npart, nullptr)], 1u);
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x)
+            if (lhist[t]) atomicAdd(&counts[t], lhist[t]);
+    } else {
+        for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
+            atomicAdd(&counts[tile_of(x, T, NT, pos, vel, i, npart, nullptr)], 1u);
+    }
+}
+
+// Exclusive scan of the tile counts by one work-group; offsets[ntiles] = npart; cursor = offsets.
+__global__ __launch_bounds__(1024) void k_tile_scan(const uint32_t *counts, int64_t ntiles,
+                                                    uint32_t *offsets, uint32_t *cursor) {
+    __shared__ uint32_t part[1024];
+    const int64_t per = ceil_div(ntiles, 1024);
+    const int64_t b = threadIdx.x * per, e = min(ntiles, b + per);
+    uint32_t s = 0;
+    for (int64_t t = b; t < e; ++t) s += counts[t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int64_t t = b; t < e; ++t) {
+        offsets[t] = run;
+        cursor[t] = run;
+        run += counts[t];
+    }
+    if (threadIdx.x == 1023) offsets[ntiles] = part[1023];
+}
+
+constexpr int kScatterPer = 8;   // particles per thread per LDS round
+
+__global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T, int NT,
+                                                               int64_t ntiles, const float *pos,
+                                                               const float *vel, int64_t npart,
+                                                               uint32_t *cursor, uint32_t *sx,
+                                                               uint32_t *sy, uint32_t *sz,
+                                                               int use_lds) {
+    extern __shared__ uint32_t lhist[];
+    const int64_t per_wg = ceil_div(npart, (int64_t)gridDim.x);
+    const int64_t beg = (int64_t)blockIdx.x * per_wg;
+    const int64_t end = min(npart, beg + per_wg);
+    if (!use_lds) {
+        for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+            uint32_t pk[3];
+            int t = tile_of(x, T, NT, pos, vel, i, npart, pk);
+            uint32_t slot = atomicAdd(&cursor[t], 1u);
+            sx[slot] = pk[0];
+            sy[slot] = pk[1];
+            sz[slot] = pk[2];
+        }
+        return;
+    }
+    const int64_t round = (int64_t)blockDim.x * kScatterPer;
+    for (int64_t r0 = beg; r0 < end; r0 += round) {
+        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
+        __syncthreads();
+        uint32_t pk[kScatterPer][3];
+        int tl[kScatterPer];
+        uint32_t rank[kScatterPer];
+#pragma unroll
+        for (int k = 0; k < kScatterPer; ++k) {
+            int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
+            tl[k] = -1;
+            if (i < end) {
+                tl[k] = tile_of(x, T, NT, pos, vel, i, npart, pk[k]);
+                rank[k] = atomicAdd(&lhist[tl[k]], 1u);
+            }
+        }
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
+            uint32_t c = lhist[t];
+            if (c) lhist[t] = atomicAdd(&cursor[t], c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kScatterPer; ++k) {
+            if (tl[k] >= 0) {
+                uint32_t slot = lhist[tl[k]] + rank[k];
+                sx[slot] = pk[k][0];
+                sy[slot] = pk[k][1];
+                sz[slot] = pk[k][2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ deposit
+
+// Per-dimension stencil start (relative to the tile origin, before the halo offset) and
+// weights, for one grid of the given order (see the table at the top of this file).
+template <int ORDER>
+__device__ __forceinline__ void stencil(int grid_sel, int single, int c, float f, int &start,
+                                        float w[4]) {
+    if (ORDER == 4) {
+        float h;
+        if (grid_sel == 0) {
+            start = c - 1;
+            h = f;
+        } else if (f >= 0.5f) {
+            start = c - 1;
+            h = f - 0.5f;
+        } else {
+            start = c - 2;
+            h = f + 0.5f;
+        }
+        float t = 1.0f - h, h2 = h * h;
+        w[0] = (t * t) * t * (1.0f / 6.0f);
+        w[1] = 2.0f / 3.0f + (0.5f * h - 1.0f) * h2;
+        w[3] = h2 * h * (1.0f / 6.0f);
+        w[2] = 1.0f - w[0] - w[1] - w[3];
+    } else if (ORDER == 3) {
+        float h;
+        if (grid_sel == 0) {
+            int up = f >= 0.5f;
+            start = c - 1 + up;
+            h = f - (float)up;
+        } else {
+            start = c - 1;
+            h = f - 0.5f;
+        }
+        float t = 0.5f - h;
+        w[0] = 0.5f * (t * t);
+        w[1] = 0.75f - h * h;
+        w[2] = 1.0f - w[0] - w[1];
+        w[3] = 0.0f;
+    } else if (ORDER == 2) {
+        float h;
+        if (single) {
+            start = c + 1;
+            h = f - (f >= 0.5f ? 1.0f : 0.0f);
+        } else if (grid_sel == 0) {
+            start = c + 1;
+            h = f;
+        } else {
+            int up = f >= 0.5f;
+            start = c + up;
+            h = f + 0.5f - (float)up;
+        }
+        w[0] = 1.0f - h;
+        w[1] = h;
+        w[2] = w[3] = 0.0f;
+    } else {
+        start = grid_sel == 0 ? c + (f >= 0.5f ? 1 : 0) : c;
+        w[0] = 1.0f;
+        w[1] = w[2] = w[3] = 0.0f;
+    }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int ngrids,
+                                                             const uint32_t *offsets,
+                                                             const uint32_t *sx,
+                                                             const uint32_t *sy,
+                                                             const uint32_t *sz,
+                                                             float *regions) {
+    __shared__ unsigned long long acc[2 * kMaxR * kMaxR * kMaxR];
+    const int R3 = R * R * R;
+    const int64_t tile = blockIdx.x;
+    for (int i = threadIdx.x; i < ngrids * R3; i += blockDim.x) acc[i] = 0ull;
+    __syncthreads();
+    const uint32_t b = offsets[tile], e = offsets[tile + 1];
+    const int single = ngrids == 1;
+    for (uint32_t p = b + threadIdx.x; p < e; p += blockDim.x) {
+        uint32_t q[3] = {sx[p], sy[p], sz[p]};
+        int c[3];
+        float f[3];
+        for (int d = 0; d < 3; ++d) {
+            c[d] = (int)(q[d] >> 28);
+            f[d] = (float)(q[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
+        }
+        for (int g = 0; g < ngrids; ++g) {
+            int st[3];
+            float w[3][4];
+            for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
+            unsigned long long *a = acc + g * R3;
+#pragma unroll
+            for (int ix = 0; ix < ORDER; ++ix) {
+#pragma unroll
+                for (int iy = 0; iy < ORDER; ++iy) {
+                    const float wxy = w[0][ix] * w[1][iy];
+                    const int row = (st[1] + kHalo + iy) * R + st[0] + kHalo + ix;
+#pragma unroll
+                    for (int iz = 0; iz < ORDER; ++iz) {
+                        const float v = wxy * w[2][iz];
+                        const long long fx = __float2ll_rn(v * kFix);
+                        atomicAdd(&a[(st[2] + kHalo + iz) * R * R + row],
+                                  (unsigned long long)fx);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float *out = regions + tile * (int64_t)ngrids * R3;
+    for (int i = threadIdx.x; i < ngrids * R3; i += blockDim.x)
+        out[i] = __ll2float_rn((long long)acc[i]) * kInvFix;
+}
+
+// grid[g][z][y][x] = sum of the region entries covering (x,y,z), in a fixed order.
+__global__ void k_gather(int N, int T, int NT, int R, int ngrids, const float *regions,
+                         float *grids) {
+    const int64_t V = (int64_t)N * N * N;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= V) return;
+    const int x = (int)(idx % N), y = (int)((idx / N) % N), z = (int)(idx / ((int64_t)N * N));
+    const int pos[3] = {x, y, z};
+    int nt[3][3], ni[3][3], cnt[3];
+    for (int d = 0; d < 3; ++d) {
+        const int t = pos[d] / T, c = pos[d] - t * T;
+        int k = 0;
+        nt[d][k] = t;
+        ni[d][k++] = c + kHalo;
+        if (c < kHalo) {
+            nt[d][k] = (t + NT - 1) % NT;
+            ni[d][k++] = c + T + kHalo;
+        }
+        if (c >= T - kHalo) {
+            nt[d][k] = (t + 1) % NT;
+            ni[d][k++] = c - T + kHalo;
+        }
+        cnt[d] = k;
+    }
+    const int64_t R3 = (int64_t)R * R * R;
+    for (int g = 0; g < ngrids; ++g) {
+        float s = 0.0f;
+        for (int a = 0; a < cnt[2]; ++a)
+            for (int b = 0; b < cnt[1]; ++b)
+                for (int c = 0; c < cnt[0]; ++c) {
+                    const int64_t tile = ((int64_t)nt[2][a] * NT + nt[1][b]) * NT + nt[0][c];
+                    const int64_t o = (tile * ngrids + g) * R3 +
+                                      ((int64_t)ni[2][a] * R + ni[1][b]) * R + ni[0][c];
+                    s += regions[o];
+                }
+        grids[(int64_t)g * V + idx] = s;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+AssignPlan make_assign_plan(const pbi4_assign_desc &d) {
+    PBI4_CHECK(d.grid >= 4, PBI4_EINVAL, "grid must be >= 4");
+    PBI4_CHECK(d.order >= 1 && d.order <= 4, PBI4_EINVAL, "interpolation order must be 1..4");
+    PBI4_CHECK(d.interlacing == 0 || d.interlacing == 1, PBI4_EINVAL, "interlacing must be 0/1");
+    PBI4_CHECK(d.pos_mode == PBI4_POS_NORMALISED || d.pos_mode == PBI4_POS_BOX, PBI4_EINVAL,
+               "bad pos_mode");
+    PBI4_CHECK(d.pos_mode == PBI4_POS_NORMALISED || d.box > 0.0f, PBI4_EINVAL,
+               "box must be > 0");
+    AssignPlan p{};
+    p.N = d.grid;
+    p.T = 0;
+    for (int t = 16; t >= 4; --t)
+        if (p.N % t == 0) {
+            p.T = t;
+            break;
+        }
+    PBI4_CHECK(p.T != 0, PBI4_EUNSUPPORTED,
+               "grid " + std::to_string(p.N) + " has no divisor in [4, 16] for the tile size");
+    p.NT = p.N / p.T;
+    p.R = p.T + 2 * kHalo;
+    p.ntiles = (int64_t)p.NT * p.NT * p.NT;
+    p.order = d.order;
+    p.interlacing = d.interlacing;
+    p.ngrids = d.interlacing ? 2 : 1;
+    p.pos_mode = d.pos_mode;
+    p.box = d.box;
+    for (int i = 0; i < 3; ++i) p.rsd[i] = d.rsd[i];
+    return p;
+}
+
+size_t assign_workspace(const AssignPlan &p, int64_t npart) {
+    Arena a(nullptr, 0);
+    a.take<uint32_t>(p.ntiles + 1);     // counts
+    a.take<uint32_t>(p.ntiles + 1);     // offsets
+    a.take<uint32_t>(p.ntiles + 1);     // cursor
+    a.take<uint32_t>(npart);            // sorted x
+    a.take<uint32_t>(npart);            // sorted y
+    a.take<uint32_t>(npart);            // sorted z
+    a.take<float>((size_t)p.ntiles * p.ngrids * p.R * p.R * p.R);   // regions
+    return a.used + 256;
+}
+
+void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t npart,
+                float *grids, Arena &ws, hipStream_t s, KernelTimer *timer) {
+    PBI4_CHECK(npart >= 0 && npart < (int64_t)0xFFFFFFFFll, PBI4_EINVAL,
+               "npart must be in [0, 2^32)");
+    PBI4_CHECK(pos != nullptr || npart == 0, PBI4_EINVAL, "pos is NULL");
+    uint32_t *counts = ws.take<uint32_t>(p.ntiles + 1);
+    uint32_t *offsets = ws.take<uint32_t>(p.ntiles + 1);
+    uint32_t *cursor = ws.take<uint32_t>(p.ntiles + 1);
+    uint32_t *sx = ws.take<uint32_t>(npart);
+    uint32_t *sy = ws.take<uint32_t>(npart);
+    uint32_t *sz = ws.take<uint32_t>(npart);
+    const int R3 = p.R * p.R * p.R;
+    float *regions = ws.take<float>((size_t)p.ntiles * p.ngrids * R3);
+
+    PosXform x{};
+    x.N = p.N;
+    x.pos_mode = p.pos_mode;
+    x.box = p.box;
+    for (int i = 0; i < 3; ++i) x.rsd[i] = p.rsd[i];
+    x.has_vel = vel != nullptr && p.pos_mode == PBI4_POS_BOX;
+
+    const int use_lds = p.ntiles <= kLdsHistMax;
+    const size_t lds = use_lds ? (size_t)p.ntiles * sizeof(uint32_t) : 0;
+    if (lds > 64 * 1024) {
+        PBI4_HIP(hipFuncSetAttribute((const void *)k_tile_hist,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        PBI4_HIP(hipFuncSetAttribute((const void *)k_tile_scatter,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    const int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(npart, 8192)));
+
+    launch_fill_u32(counts, 0u, p.ntiles + 1, s);
+    if (npart > 0)
+        hipLaunchKernelGGL(k_tile_hist, dim3(nwg), dim3(kSortThreads), lds, s, x, p.T, p.NT,
+                           p.ntiles, pos, vel, npart, counts, use_lds);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, p.ntiles, offsets, cursor);
+    if (npart > 0)
+        hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(kSortThreads), lds, s, x, p.T, p.NT,
+                           p.ntiles, pos, vel, npart, cursor, sx, sy, sz, use_lds);
+    {
+        TimedScope ts(timer, "pbi4_deposit", s);
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3((unsigned)p.ntiles), dim3(kDepositThreads), 0, s, p.T,
+                               p.R, p.ngrids, offsets, sx, sy, sz, regions);
+        };
+        switch (p.order) {
+            case 1: launch(k_deposit<1>); break;
+            case 2: launch(k_deposit<2>); break;
+            case 3: launch(k_deposit<3>); break;
+            default: launch(k_deposit<4>); break;
+        }
+    }
+    const int64_t V = (int64_t)p.N * p.N * p.N;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)ceil_div(V, 256)), dim3(256), 0, s, p.N, p.T,
+                       p.NT, p.R, p.ngrids, regions, grids);
+    PBI4_HIP(hipGetLastError());
+}
+
+}  // namespace pbi4

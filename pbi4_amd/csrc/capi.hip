@@ -1,0 +1,757 @@
+// C ABI of libpbi4hip.so: context, cached tables and rocFFT plans, entry points.
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "common.h"
+
+namespace pbi4 {
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+TimedScope::TimedScope(KernelTimer *t_, const char *k, hipStream_t s_) : t(t_), s(s_) {
+    if (t && !t->name.empty() && t->name == k) {
+        hipEvent_t a;
+        if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+            (void)hipEventRecord(a, s);
+            t->events.push_back({a, b});
+        }
+    }
+}
+TimedScope::~TimedScope() {
+    if (b) (void)hipEventRecord(b, s);
+}
+
+}  // namespace pbi4
+
+using namespace pbi4;
+
+namespace {
+
+// ------------------------------------------------------------------ host-built tables
+// These follow the reference formulas operation by operation in FP64 (no FMA contraction), so
+// mode-to-bin and mode-to-shell assignment is bit-exact, ties included.
+#pragma clang fp contract(off)
+
+// P(k) bin of (|kx|, |ky|, |kz|): imk = nint(rk/dk - kcenter/kbin - tiny) + 1, with
+// kF = REAL(4) twopi/box and float(ik) (powerI4.f90:527-566).  0 = outside 1..nbins.
+std::vector<uint16_t> build_pk_table(int N, float box, float kbin, float kcenter, int nbins) {
+    const int h = N / 2 + 1;
+    const double kF = (double)(kTwoPiF32 / box);
+    const double dk = (double)kbin * kF;
+    const double kcb = (double)(kcenter / kbin);
+    std::vector<double> sq(h);
+    for (int a = 0; a < h; ++a) {
+        const double r = kF * (double)(float)a;
+        sq[a] = r * r;
+    }
+    std::vector<uint16_t> t((size_t)h * h * h);
+    for (int a = 0; a < h; ++a)
+        for (int b = 0; b < h; ++b)
+            for (int c = 0; c < h; ++c) {
+                const double rk = std::sqrt(sq[a] + sq[b] + sq[c]);
+                const long imk = std::lround(rk / dk - kcb - DBL_MIN) + 1;
+                t[((size_t)a * h + b) * h + c] = (imk >= 1 && imk <= nbins) ? (uint16_t)imk : 0;
+            }
+    return t;
+}
+
+// Bispectrum shell of a grid point with per-axis distances d = min(i, N-i):
+// idist = nint(sqrt(d2)/kbin - kcenter/kbin - tiny) + 1 (bispectrumI4.f90:69-73).
+int shell_of_d2(long d2, float kbin, float kcenter) {
+    const double kcb = (double)(kcenter / kbin);
+    return (int)(std::lround(std::sqrt((double)d2) / (double)kbin - kcb - DBL_MIN) + 1);
+}
+
+#pragma clang fp contract(on)
+
+struct ShellInfo {
+    int N, nb, max_d2, R;
+    std::vector<int16_t> table;   // d2 -> shell (0 = none)
+    std::vector<double> nmodes;   // full-grid modes per shell
+    int16_t *table_d = nullptr;
+    double *nmodes_d = nullptr;
+};
+
+struct PkTable {
+    uint16_t *d = nullptr;
+    std::vector<double> nmodes;
+};
+
+struct FftPlan {
+    rocfft_plan plan = nullptr;
+    size_t work = 0;
+};
+
+struct CountsCache {
+    std::vector<unsigned long long> T;   // (nb+1)^3
+};
+
+}  // namespace
+
+struct pbi4_ctx {
+    int device = 0;
+    std::string err;
+    std::mutex mu;
+    KernelTimer timer;
+    std::map<std::tuple<int, float, float, float, int>, PkTable> pk_tables;
+    std::map<std::tuple<int, int, float, float>, std::unique_ptr<ShellInfo>> shells;
+    std::map<std::tuple<int, int>, float *> windows;   // (N, order) -> device sinc^p table
+    std::map<std::tuple<int, int, int>, FftPlan> plans; // (kind, N, batch/ext)
+    std::map<std::tuple<int, int, float, float>, CountsCache> counts;
+    std::map<std::tuple<int, int, float, float, int, int>, int32_t *> tri_tables;
+    std::vector<void *> owned;   // device allocations owned by the context
+};
+
+namespace {
+
+int set_err(pbi4_ctx *ctx, const Error &e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+}
+
+template <class F>
+int guarded(pbi4_ctx *ctx, F &&f) {
+    try {
+        if (ctx) {
+            PBI4_HIP(hipSetDevice(ctx->device));
+        }
+        f();
+        return PBI4_OK;
+    } catch (const Error &e) {
+        return set_err(ctx, e);
+    } catch (const std::exception &e) {
+        return set_err(ctx, Error{PBI4_EINVAL, e.what()});
+    }
+}
+
+template <class T>
+T *dev_upload(pbi4_ctx *ctx, const std::vector<T> &v) {
+    T *d = nullptr;
+    PBI4_HIP(hipMalloc(&d, std::max<size_t>(1, v.size()) * sizeof(T)));
+    ctx->owned.push_back(d);
+    if (!v.empty()) PBI4_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+void fft_check(rocfft_status st, const char *what) {
+    if (st != rocfft_status_success)
+        fail(PBI4_EFFT, std::string(what) + " failed (rocfft status " + std::to_string((int)st) + ")");
+}
+
+struct RocfftInit {
+    RocfftInit() { rocfft_setup(); }
+};
+
+// kind 0: r2c of N^3 with `batch` grids (default layouts).
+// kind 1: c2r of N^3 with `batch` maps, real output per-map distance `ext` floats.
+FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
+    static RocfftInit init;
+    auto key = std::make_tuple(kind * 1000000 + batch, N, (int)ext);
+    auto it = ctx->plans.find(key);
+    if (it != ctx->plans.end()) return it->second;
+    FftPlan p;
+    size_t len[3] = {(size_t)N, (size_t)N, (size_t)N};
+    rocfft_plan_description desc = nullptr;
+    fft_check(rocfft_plan_description_create(&desc), "rocfft_plan_description_create");
+    if (kind == 1) {
+        const size_t h = N / 2 + 1;
+        size_t in_strides[3] = {1, h, h * N};
+        size_t out_strides[3] = {1, (size_t)N, (size_t)N * N};
+        fft_check(rocfft_plan_description_set_data_layout(
+                      desc, rocfft_array_type_hermitian_interleaved, rocfft_array_type_real, nullptr,
+                      nullptr, 3, in_strides, h * N * N, 3, out_strides, (size_t)ext),
+                  "rocfft_plan_description_set_data_layout");
+    }
+    fft_check(rocfft_plan_create(&p.plan, rocfft_placement_notinplace,
+                                 kind == 0 ? rocfft_transform_type_real_forward
+                                           : rocfft_transform_type_real_inverse,
+                                 rocfft_precision_single, 3, len, (size_t)batch, desc),
+              "rocfft_plan_create");
+    rocfft_plan_description_destroy(desc);
+    fft_check(rocfft_plan_get_work_buffer_size(p.plan, &p.work), "rocfft_plan_get_work_buffer_size");
+    return ctx->plans[key] = p;
+}
+
+void fft_exec(FftPlan &p, void *in, void *out, void *work, hipStream_t s) {
+    rocfft_execution_info info = nullptr;
+    fft_check(rocfft_execution_info_create(&info), "rocfft_execution_info_create");
+    fft_check(rocfft_execution_info_set_stream(info, s), "rocfft_execution_info_set_stream");
+    if (p.work) fft_check(rocfft_execution_info_set_work_buffer(info, work, p.work),
+                          "rocfft_execution_info_set_work_buffer");
+    void *ib[1] = {in}, *ob[1] = {out};
+    const rocfft_status st = rocfft_execute(p.plan, ib, ob, info);
+    rocfft_execution_info_destroy(info);
+    fft_check(st, "rocfft_execute");
+}
+
+// tWk of fcomb (powerI4.f90:278-292), evaluated in FP64 like the reference (x**nintp as
+// repeated squaring), stored as float.
+float *get_window(pbi4_ctx *ctx, int N, int order) {
+    auto key = std::make_tuple(N, order);
+    auto it = ctx->windows.find(key);
+    if (it != ctx->windows.end()) return it->second;
+    const int h = N / 2 + 1;
+    std::vector<float> w(h);
+    const double tpiN = (double)kTwoPiF32 / (double)N;
+    w[0] = 1.0f;
+    for (int i = 1; i < h; ++i) {
+        const double rk = tpiN * (double)i;
+        double x = std::sin(rk / 2.0) / (rk / 2.0);
+        unsigned n = (unsigned)order;
+        double y = (n % 2) ? x : 1.0;
+        while (n >>= 1) {
+            x = x * x;
+            if (n % 2) y = y * x;
+        }
+        w[i] = (float)y;
+    }
+    return ctx->windows[key] = dev_upload(ctx, w);
+}
+
+// kF3 = twopi*twopi*twopi/box/box/box in REAL(4) (powerI4.f90:271)
+double kf3_fcomb(float box) {
+    volatile float t = kTwoPiF32 * kTwoPiF32;
+    t = t * kTwoPiF32;
+    t = t / box;
+    t = t / box;
+    t = t / box;
+    return (double)t;
+}
+
+PkTable &get_pk_table(pbi4_ctx *ctx, int N, float box, float kbin, float kcenter, int nbins) {
+    auto key = std::make_tuple(N, box, kbin, kcenter, nbins);
+    auto it = ctx->pk_tables.find(key);
+    if (it != ctx->pk_tables.end()) return it->second;
+    PBI4_CHECK(nbins >= 1 && nbins < 65535, PBI4_EINVAL, "nbins must be in [1, 65535)");
+    PBI4_CHECK(kbin > 0.0f, PBI4_EINVAL, "kbin must be > 0");
+    PkTable t;
+    t.d = dev_upload(ctx, build_pk_table(N, box, kbin, kcenter, nbins));
+    return ctx->pk_tables[key] = t;
+}
+
+ShellInfo &get_shells(pbi4_ctx *ctx, int N, int nb, float kbin, float kcenter) {
+    auto key = std::make_tuple(N, nb, kbin, kcenter);
+    auto it = ctx->shells.find(key);
+    if (it != ctx->shells.end()) return *it->second;
+    PBI4_CHECK(nb >= 1 && nb <= 32000, PBI4_EINVAL, "nbins must be in [1, 32000]");
+    PBI4_CHECK(kbin > 0.0f, PBI4_EINVAL, "kbin must be > 0");
+    auto s = std::make_unique<ShellInfo>();
+    s->N = N;
+    s->nb = nb;
+    const int hN = N / 2;
+    s->max_d2 = 3 * hN * hN;
+    s->table.assign(s->max_d2 + 1, 0);
+    int d2max_in = -1;
+    for (long d2 = 0; d2 <= s->max_d2; ++d2) {
+        const int sh = shell_of_d2(d2, kbin, kcenter);
+        if (sh >= 1 && sh <= nb) {
+            s->table[d2] = (int16_t)sh;
+            d2max_in = (int)d2;
+        }
+    }
+    // largest |component| of any shell mode (bounded by the per-axis distance N/2)
+    s->R = d2max_in < 0 ? 0 : std::min(hN, (int)std::floor(std::sqrt((double)d2max_in) + 1e-9));
+    // full-grid modes per shell: per-axis multiplicity of each distance v = min(i, N-i)
+    std::vector<double> mult(hN + 1, 0.0);
+    for (int i = 0; i < N; ++i) mult[std::min(i, N - i)] += 1.0;
+    s->nmodes.assign(nb, 0.0);
+    for (int a = 0; a <= hN; ++a)
+        for (int b = 0; b <= hN; ++b) {
+            const long ab = (long)a * a + (long)b * b;
+            if (ab > d2max_in) break;
+            for (int c = 0; c <= hN; ++c) {
+                const long d2 = ab + (long)c * c;
+                if (d2 > d2max_in) break;
+                const int sh = s->table[d2];
+                if (sh) s->nmodes[sh - 1] += mult[a] * mult[b] * mult[c];
+            }
+        }
+    s->table_d = dev_upload(ctx, s->table);
+    s->nmodes_d = dev_upload(ctx, s->nmodes);
+    auto &ref = *s;
+    ctx->shells[key] = std::move(s);
+    return ref;
+}
+
+bool smooth235(int n) {
+    for (int p : {2, 3, 5})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// Smallest even 2,3,5-smooth size N' with N' > 3R (no aliasing of q1+q2+q3 on N'), or N when
+// the reference's own mod-N closure aliases (3R >= N) or no smaller size exists.
+int choose_maps_grid(int N, int R) {
+    if (3 * R >= N) return N;
+    for (int n = 3 * R + 1; n < N; ++n)
+        if (n % 2 == 0 && n >= 4 && smooth235(n)) return n;
+    return N;
+}
+
+std::vector<int32_t> triangle_list(int nb, float kbin, float kcenter, int iopen, int cross) {
+    const int ishift = (int)(kcenter / kbin) + (iopen ? 1 : 0);
+    std::vector<int32_t> t;
+    for (int l = 1; l <= nb; ++l)
+        for (int j = 1; j <= l; ++j) {
+            const int imax = cross ? std::min(nb, j + l - 1 + ishift) : j;
+            for (int i = std::max(1, l - j + 1 - ishift); i <= imax; ++i) {
+                t.push_back(l);
+                t.push_back(j);
+                t.push_back(i);
+            }
+        }
+    return t;
+}
+
+int32_t *get_tri_table(pbi4_ctx *ctx, int nb, float kbin, float kcenter, int iopen, int cross,
+                       int64_t itot) {
+    auto key = std::make_tuple(nb, 0, kbin, kcenter, iopen, cross);
+    auto it = ctx->tri_tables.find(key);
+    std::vector<int32_t> t = triangle_list(nb, kbin, kcenter, iopen, cross);
+    PBI4_CHECK((int64_t)t.size() / 3 == itot, PBI4_EINVAL,
+               "itot does not match the triangle loop (" + std::to_string(t.size() / 3) + ")");
+    if (it != ctx->tri_tables.end()) return it->second;
+    return ctx->tri_tables[key] = dev_upload(ctx, t);
+}
+
+// ------------------------------------------------------------------ workspace layouts
+
+struct DensityWs {
+    float2 *G;
+    void *fftwork;
+};
+DensityWs density_layout(pbi4_ctx *ctx, const pbi4_density_desc &d, Arena &a) {
+    const int ng = d.interlacing ? 2 : 1;
+    FftPlan &p = get_plan(ctx, 0, d.grid, ng, 0);
+    DensityWs w;
+    w.G = a.take<float2>((size_t)ng * d.grid * d.grid * (d.grid / 2 + 1));
+    w.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
+    return w;
+}
+
+void check_density(const pbi4_density_desc &d) {
+    PBI4_CHECK(d.grid >= 4 && d.grid % 2 == 0, PBI4_EINVAL, "grid must be even and >= 4");
+    PBI4_CHECK(d.order >= 1 && d.order <= 4, PBI4_EINVAL, "interpolation order must be 1..4");
+    PBI4_CHECK(d.box > 0.0f, PBI4_EINVAL, "box must be > 0");
+    PBI4_CHECK(d.npart > 0.0, PBI4_EINVAL, "npart must be > 0");
+}
+
+BinPlan make_bin_plan(int N, int nbins_pk, int nshell, int multipoles, const double *los, float box) {
+    BinPlan b{};
+    b.N = N;
+    b.nbins_pk = nbins_pk;
+    b.nshell = nshell;
+    b.multipoles = multipoles;
+    for (int i = 0; i < 3; ++i) b.los[i] = (float)los[i];
+    b.kF = (float)(double)(kTwoPiF32 / box);
+    const int64_t rows = (int64_t)N * N;
+    b.nwg = (int)std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(rows, 16)));
+    return b;
+}
+
+struct BkLayout {
+    int Np, nell, nmaps;
+    int64_t K, Kstride;
+    GemmPlan g;
+    float *bin_partials;
+    double *shell_sum;
+    float *scale_exp;
+    float2 *spectra;
+    float *maps;
+    void *fftwork;
+    float *gemm_partials;
+    double *C;
+};
+
+void check_bk(const pbi4_bk_desc &d) {
+    PBI4_CHECK(d.grid >= 4 && d.grid % 2 == 0, PBI4_EINVAL, "grid must be even and >= 4");
+    PBI4_CHECK(d.nbins >= 1, PBI4_EINVAL, "nbins must be >= 1");
+    PBI4_CHECK(d.kbin > 0.0f, PBI4_EINVAL, "kbin must be > 0");
+    PBI4_CHECK(d.npart > 0, PBI4_EINVAL, "npart must be > 0");
+    PBI4_CHECK(d.precision == PBI4_PREC_FP32 || d.precision == PBI4_PREC_BF16X3, PBI4_EINVAL,
+               "precision must be PBI4_PREC_FP32 or PBI4_PREC_BF16X3");
+    PBI4_CHECK(d.maps_grid == 0 || (d.maps_grid >= 4 && d.maps_grid % 2 == 0 && d.maps_grid <= d.grid),
+               PBI4_EINVAL, "maps_grid must be 0 or an even size <= grid");
+}
+
+BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
+    ShellInfo &sh = get_shells(ctx, d.grid, d.nbins, d.kbin, d.kcenter);
+    BkLayout L{};
+    L.Np = d.maps_grid ? d.maps_grid : choose_maps_grid(d.grid, sh.R);
+    PBI4_CHECK(d.maps_grid == 0 || L.Np == d.grid || 3 * sh.R < L.Np, PBI4_EINVAL,
+               "maps_grid too small for the shells (needs > 3 * max|q| = " +
+                   std::to_string(3 * sh.R) + ")");
+    L.nell = d.multipoles ? 3 : 1;
+    L.nmaps = d.nbins * L.nell;
+    L.K = (int64_t)L.Np * L.Np * L.Np;
+    L.Kstride = (int64_t)align_up((size_t)L.K, 64);
+    L.g = make_gemm_plan(d.nbins, L.nell, L.K, L.Kstride, d.precision);
+    BinPlan b = make_bin_plan(d.grid, 0, d.nbins, 0, d.los, 1.0f);
+    FftPlan &p = get_plan(ctx, 1, L.Np, L.nmaps, L.Kstride);
+    L.bin_partials = a.take<float>(bin_partials_floats(b));
+    L.shell_sum = a.take<double>(d.nbins);
+    L.scale_exp = a.take<float>(d.nbins);
+    L.spectra = a.take<float2>((size_t)L.nmaps * L.Np * L.Np * (L.Np / 2 + 1));
+    L.maps = a.take<float>((size_t)L.nmaps * L.Kstride);
+    L.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
+    L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
+    L.C = a.take<double>((size_t)L.g.npair * L.g.ncol);
+    return L;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+
+extern "C" {
+
+int pbi4_version(void) { return 100; }
+
+int pbi4_ctx_create(int device, pbi4_ctx **out) {
+    if (!out) return PBI4_EINVAL;
+    auto *c = new pbi4_ctx();
+    c->device = device;
+    int rc = guarded(c, [&] {
+        int n = 0;
+        PBI4_HIP(hipGetDeviceCount(&n));
+        PBI4_CHECK(device >= 0 && device < n, PBI4_EINVAL, "no such device");
+    });
+    if (rc != PBI4_OK) {
+        delete c;
+        *out = nullptr;
+        return rc;
+    }
+    *out = c;
+    return PBI4_OK;
+}
+
+int pbi4_ctx_destroy(pbi4_ctx *ctx) {
+    if (!ctx) return PBI4_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : ctx->plans)
+        if (kv.second.plan) rocfft_plan_destroy(kv.second.plan);
+    for (void *p : ctx->owned) (void)hipFree(p);
+    for (auto &e : ctx->timer.events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    delete ctx;
+    return PBI4_OK;
+}
+
+const char *pbi4_last_error(const pbi4_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+size_t pbi4_assign_workspace(const pbi4_assign_desc *d, int64_t npart) {
+    try {
+        return assign_workspace(make_assign_plan(*d), npart);
+    } catch (...) {
+        return 0;
+    }
+}
+
+int pbi4_assign(pbi4_ctx *ctx, const pbi4_assign_desc *d, const float *pos, const float *vel,
+                int64_t npart, float *grids, void *ws, size_t ws_bytes, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(d && grids && ws, PBI4_EINVAL, "NULL argument");
+        AssignPlan p = make_assign_plan(*d);
+        Arena a(ws, ws_bytes);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        assign_run(p, pos, vel, npart, grids, a, (hipStream_t)stream, &ctx->timer);
+    });
+}
+
+size_t pbi4_density_workspace(pbi4_ctx *ctx, const pbi4_density_desc *d) {
+    size_t r = 0;
+    guarded(ctx, [&] {
+        check_density(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        Arena a(nullptr, 0);
+        density_layout(ctx, *d, a);
+        r = a.used + 256;
+    });
+    return r;
+}
+
+int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float *grids,
+                         void *delta, void *ws, size_t ws_bytes, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(d && grids && delta && ws, PBI4_EINVAL, "NULL argument");
+        check_density(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipStream_t s = (hipStream_t)stream;
+        Arena a(ws, ws_bytes);
+        DensityWs w = density_layout(ctx, *d, a);
+        const int ng = d->interlacing ? 2 : 1;
+        FftPlan &p = get_plan(ctx, 0, d->grid, ng, 0);
+        {
+            TimedScope ts(&ctx->timer, "pbi4_r2c", s);
+            fft_exec(p, const_cast<float *>(grids), w.G, w.fftwork, s);
+        }
+        const double kF3 = kf3_fcomb(d->box);
+        const double cf = (d->interlacing ? 1.0 / (4.0 * kF3) : 1.0 / kF3) / d->npart;
+        const int64_t half = (int64_t)d->grid * d->grid * (d->grid / 2 + 1);
+        TimedScope ts(&ctx->timer, "pbi4_combine", s);
+        launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order), (float)cf, w.G,
+                       w.G + half, static_cast<float2 *>(delta), s);
+    });
+}
+
+size_t pbi4_pk_workspace(pbi4_ctx *ctx, const pbi4_pk_desc *d) {
+    (void)ctx;
+    BinPlan b = make_bin_plan(d->grid, d->nbins, 0, d->multipoles, d->los, d->box > 0 ? d->box : 1.0f);
+    return align_up(bin_partials_floats(b) * sizeof(float), 256) + 256;
+}
+
+int pbi4_measure_pk(pbi4_ctx *ctx, const pbi4_pk_desc *d, const void *delta, const void *delta_b,
+                    const void *delta2, const void *delta4, double *pout, void *ws,
+                    size_t ws_bytes, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(d && delta && pout && ws, PBI4_EINVAL, "NULL argument");
+        PBI4_CHECK(d->grid >= 4 && d->grid % 2 == 0, PBI4_EINVAL, "grid must be even and >= 4");
+        PBI4_CHECK(d->box > 0.0f, PBI4_EINVAL, "box must be > 0");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipStream_t s = (hipStream_t)stream;
+        PkTable &t = get_pk_table(ctx, d->grid, d->box, d->kbin, d->kcenter, d->nbins);
+        BinPlan b = make_bin_plan(d->grid, d->nbins, 0, d->multipoles, d->los, d->box);
+        b.pk_table = t.d;
+        b.cross = static_cast<const float2 *>(delta_b);
+        b.f2 = static_cast<const float2 *>(delta2);
+        b.f4 = static_cast<const float2 *>(delta4);
+        PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
+                   "delta2/delta4 are only used with multipoles");
+        PBI4_CHECK(!d->multipoles || !delta_b, PBI4_EINVAL,
+                   "a cross spectrum is real-space only (pbi4/pbi4.py:434-436)");
+        Arena a(ws, ws_bytes);
+        float *part = a.take<float>(bin_partials_floats(b));
+        const double kF = (double)(kTwoPiF32 / d->box);
+        {
+            TimedScope ts(&ctx->timer, "pbi4_bin", s);
+            launch_bin(b, static_cast<const float2 *>(delta), part, s);
+        }
+        launch_bin_finalize(b, part, kF * kF * kF, pout, nullptr, s);
+    });
+}
+
+int pbi4_count_triangles(pbi4_ctx *ctx, int nbins, int grid, float kbin, float kcenter, int iopen,
+                         int cross, int64_t itot, int64_t *ntr) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(ntr, PBI4_EINVAL, "NULL argument");
+        PBI4_CHECK(grid >= 4 && grid % 2 == 0, PBI4_EINVAL, "grid must be even and >= 4");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        std::vector<int32_t> tri = triangle_list(nbins, kbin, kcenter, iopen, cross);
+        PBI4_CHECK((int64_t)tri.size() / 3 == itot, PBI4_EINVAL,
+                   "itot does not match the triangle loop (" + std::to_string(tri.size() / 3) + ")");
+        ShellInfo &sh = get_shells(ctx, grid, nbins, kbin, kcenter);
+        auto key = std::make_tuple(grid, nbins, kbin, kcenter);
+        auto it = ctx->counts.find(key);
+        if (it == ctx->counts.end()) {
+            const int N = grid, hN = N / 2, S = nbins + 1;
+            // residues per axis with distance <= R
+            std::vector<int> ax;
+            for (int i = 0; i < N; ++i)
+                if (std::min(i, N - i) <= sh.R) ax.push_back(i <= hN ? i : i - N);
+            std::vector<std::vector<int32_t>> per(nbins);
+            for (int x : ax)
+                for (int y : ax)
+                    for (int z : ax) {
+                        const long d2 = (long)x * x + (long)y * y + (long)z * z;
+                        const int s = d2 <= sh.max_d2 ? sh.table[d2] : 0;
+                        if (s) {
+                            auto &v = per[s - 1];
+                            v.push_back(x);
+                            v.push_back(y);
+                            v.push_back(z);
+                            v.push_back(s);
+                        }
+                    }
+            std::vector<int32_t> modes;
+            std::vector<int64_t> off(nbins + 1, 0);
+            for (int s = 0; s < nbins; ++s) {
+                off[s] = (int64_t)modes.size() / 4;
+                modes.insert(modes.end(), per[s].begin(), per[s].end());
+            }
+            off[nbins] = (int64_t)modes.size() / 4;
+            // canonical representatives a >= b >= c >= 0 and their orbit sizes
+            std::vector<int32_t> canon, w;
+            for (int a0 = 0; a0 <= std::min(sh.R, hN); ++a0)
+                for (int b0 = 0; b0 <= a0; ++b0)
+                    for (int c0 = 0; c0 <= b0; ++c0) {
+                        const long d2 = (long)a0 * a0 + (long)b0 * b0 + (long)c0 * c0;
+                        const int s = d2 <= sh.max_d2 ? sh.table[d2] : 0;
+                        if (!s) continue;
+                        int perms = (a0 == b0 && b0 == c0) ? 1 : (a0 == b0 || b0 == c0) ? 3 : 6;
+                        int signs = 1;
+                        for (int v : {a0, b0, c0})
+                            if (v != 0 && 2 * v != N) signs *= 2;
+                        canon.insert(canon.end(), {a0, b0, c0, s});
+                        w.push_back(perms * signs);
+                    }
+            std::vector<int32_t> modes_v = modes;
+            int32_t *modes_d = dev_upload(ctx, modes_v);
+            int64_t *off_d = dev_upload(ctx, off);
+            int32_t *canon_d = dev_upload(ctx, canon);
+            int32_t *w_d = dev_upload(ctx, w);
+            unsigned long long *T_d = nullptr;
+            const size_t nT = (size_t)S * S * S;
+            PBI4_HIP(hipMalloc(&T_d, nT * sizeof(unsigned long long)));
+            PBI4_HIP(hipMemset(T_d, 0, nT * sizeof(unsigned long long)));
+            count_pairs_gpu(N, nbins, modes_d, off[nbins], off_d, canon_d, w_d, (int64_t)w.size(),
+                            sh.table_d, sh.max_d2, T_d, nullptr);
+            CountsCache cc;
+            cc.T.resize(nT);
+            PBI4_HIP(hipMemcpy(cc.T.data(), T_d, nT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            PBI4_HIP(hipFree(T_d));
+            it = ctx->counts.emplace(key, std::move(cc)).first;
+        }
+        const int S = nbins + 1;
+        for (int64_t m = 0; m < itot; ++m) {
+            const int l = tri[3 * m], j = tri[3 * m + 1], i = tri[3 * m + 2];
+            const int a = std::min(i, j), b = std::max(i, j);
+            ntr[m] = (int64_t)it->second.T[((size_t)a * S + b) * S + l];
+        }
+    });
+}
+
+int pbi4_bk_maps_grid(pbi4_ctx *ctx, const pbi4_bk_desc *d) {
+    int r = 0;
+    int rc = guarded(ctx, [&] {
+        check_bk(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ShellInfo &sh = get_shells(ctx, d->grid, d->nbins, d->kbin, d->kcenter);
+        r = d->maps_grid ? d->maps_grid : choose_maps_grid(d->grid, sh.R);
+    });
+    return rc == PBI4_OK ? r : rc;
+}
+
+size_t pbi4_bk_workspace(pbi4_ctx *ctx, const pbi4_bk_desc *d) {
+    size_t r = 0;
+    guarded(ctx, [&] {
+        check_bk(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        Arena a(nullptr, 0);
+        bk_layout(ctx, *d, a);
+        r = a.used + 256;
+    });
+    return r;
+}
+
+int pbi4_measure_bk(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *counts, const void *delta,
+                    const void *delta2, const void *delta4, double *bout, void *ws,
+                    size_t ws_bytes, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(d && counts && delta && bout && ws, PBI4_EINVAL, "NULL argument");
+        check_bk(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipStream_t s = (hipStream_t)stream;
+        ShellInfo &sh = get_shells(ctx, d->grid, d->nbins, d->kbin, d->kcenter);
+        int32_t *tri = get_tri_table(ctx, d->nbins, d->kbin, d->kcenter, d->iopen, 0, d->itot);
+        Arena a(ws, ws_bytes);
+        BkLayout L = bk_layout(ctx, *d, a);
+        // shell powers (Parseval) for pow(s) and the map scales
+        BinPlan b = make_bin_plan(d->grid, 0, d->nbins, 0, d->los, 1.0f);
+        b.shell_table = sh.table_d;
+        b.max_d2 = sh.max_d2;
+        launch_bin(b, static_cast<const float2 *>(delta), L.bin_partials, s);
+        launch_bin_finalize(b, L.bin_partials, 1.0, nullptr, L.shell_sum, s);
+        FilterPlan f{};
+        f.N = d->grid;
+        f.Np = L.Np;
+        f.R = sh.R;
+        f.nshell = d->nbins;
+        f.nell = L.nell;
+        for (int i = 0; i < 3; ++i) f.los[i] = (float)d->los[i];
+        f.shell_table = sh.table_d;
+        f.max_d2 = sh.max_d2;
+        f.f2 = static_cast<const float2 *>(delta2);
+        f.f4 = static_cast<const float2 *>(delta4);
+        PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
+                   "delta2/delta4 are only used with multipoles");
+        {
+            TimedScope ts(&ctx->timer, "pbi4_filter", s);
+            launch_filter(f, static_cast<const float2 *>(delta), L.shell_sum, sh.nmodes_d, L.spectra,
+                          L.scale_exp, s);
+        }
+        FftPlan &p = get_plan(ctx, 1, L.Np, L.nmaps, L.Kstride);
+        {
+            TimedScope ts(&ctx->timer, "pbi4_c2r", s);
+            fft_exec(p, L.spectra, L.maps, L.fftwork, s);
+        }
+        launch_gemm(L.g, L.maps, L.gemm_partials, s, &ctx->timer);
+        launch_gemm_reduce(L.g, L.gemm_partials, L.C, s);
+        // kf3 = kf**3 in REAL(4) (bispectrumI4.f90:46); powSN = 1/dble(npart)/kf3 (:47)
+        volatile float kfv = d->kf;
+        volatile float k2 = kfv * kfv;
+        volatile float k3 = k2 * kfv;
+        const double kf3 = (double)k3;
+        const double powSN = 1.0 / (double)d->npart / kf3;
+        const double gridD = (double)d->grid * d->grid * d->grid;
+        const double ratio = (double)d->grid / (double)L.Np;
+        launch_bk_finalize(d->multipoles ? 11 : 9, d->itot, tri, L.C, L.g.npair, L.g.ncol, d->nbins,
+                           L.scale_exp, L.shell_sum, sh.nmodes_d, counts, kf3, powSN, gridD,
+                           ratio * ratio * ratio, (double)d->kcenter, (double)d->kbin, bout, s);
+    });
+}
+
+size_t pbi4_triangle_sums_workspace(int nb, int nell, int64_t K, int64_t Kstride, int precision) {
+    try {
+        GemmPlan g = make_gemm_plan(nb, nell, K, Kstride, precision);
+        return align_up(gemm_partials_floats(g) * sizeof(float), 256) + 256;
+    } catch (...) {
+        return 0;
+    }
+}
+
+int pbi4_triangle_sums(pbi4_ctx *ctx, int nb, int nell, int64_t K, int64_t Kstride, int precision,
+                       const float *maps, double *C, void *ws, size_t ws_bytes, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(maps && C && ws, PBI4_EINVAL, "NULL argument");
+        PBI4_CHECK(nb >= 1 && (nell == 1 || nell == 3), PBI4_EINVAL, "bad nb / nell");
+        PBI4_CHECK(Kstride >= K && K > 0, PBI4_EINVAL, "bad K / Kstride");
+        PBI4_CHECK(precision == PBI4_PREC_FP32 || precision == PBI4_PREC_BF16X3, PBI4_EINVAL,
+                   "bad precision");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        GemmPlan gp = make_gemm_plan(nb, nell, K, Kstride, precision);
+        Arena a(ws, ws_bytes);
+        float *part = a.take<float>(gemm_partials_floats(gp));
+        launch_gemm(gp, maps, part, (hipStream_t)stream, &ctx->timer);
+        launch_gemm_reduce(gp, part, C, (hipStream_t)stream);
+    });
+}
+
+int pbi4_timing_enable(pbi4_ctx *ctx, const char *kernel_name) {
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        for (auto &e : ctx->timer.events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        ctx->timer.events.clear();
+        ctx->timer.name = kernel_name ? kernel_name : "";
+    });
+}
+
+int pbi4_timing_read(pbi4_ctx *ctx, double *total_ms, int64_t *launches) {
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        double t = 0.0;
+        for (auto &e : ctx->timer.events) {
+            PBI4_HIP(hipEventSynchronize(e.second));
+            float ms = 0.0f;
+            PBI4_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+            t += ms;
+        }
+        if (total_ms) *total_ms = t;
+        if (launches) *launches = (int64_t)ctx->timer.events.size();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// Internal definitions shared by the HIP translation units of libpbi4hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pbi4_amd.h"
+
+namespace pbi4 {
+
+struct KernelTimer;
+
+// REAL(4), parameter :: twopi = 6.28319 of the reference (powerI4.f90:72 and every routine).
+constexpr float kTwoPiF32 = 6.28319f;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+// Throwing helpers: every C entry point catches and converts to a status code.
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define PBI4_HIP(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            ::pbi4::fail(PBI4_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+#define PBI4_CHECK(cond, code, msg)                                                        \
+    do {                                                                                   \
+        if (!(cond)) ::pbi4::fail((code), (msg));                                          \
+    } while (0)
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+// Bump allocator over a caller-provided device workspace.
+struct Arena {
+    char *base;
+    size_t size, used;
+    Arena(void *b, size_t s) : base(static_cast<char *>(b)), size(s), used(0) {}
+    template <class T>
+    T *take(size_t count) {
+        size_t off = align_up(used, 256);
+        size_t bytes = align_up(count * sizeof(T), 256);
+        if (base != nullptr && off + bytes > size)
+            fail(PBI4_ENOMEM, "workspace too small: need " + std::to_string(off + bytes) +
+                                  " bytes, have " + std::to_string(size));
+        used = off + bytes;
+        return base ? reinterpret_cast<T *>(base + off) : nullptr;
+    }
+};
+
+// ---------------------------------------------------------------- assignment (assign.hip)
+struct AssignPlan {
+    int N, T, NT, R, order, interlacing, ngrids, pos_mode;
+    int64_t ntiles;
+    float box, rsd[3];
+};
+AssignPlan make_assign_plan(const pbi4_assign_desc &d);
+size_t assign_workspace(const AssignPlan &p, int64_t npart);
+void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t npart,
+                float *grids, Arena &ws, hipStream_t s, KernelTimer *timer);
+
+// ---------------------------------------------------------------- spectra (spectrum.hip)
+void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
+                    const float2 *G2, float2 *delta, hipStream_t s);
+
+struct BinPlan {
+    int N;                    // grid
+    int nbins_pk;             // P(k) bins (0 = none)
+    int nshell;               // bispectrum shells (0 = none)
+    int multipoles;
+    float los[3];
+    float kF;                 // |k| scale for avgk
+    const uint16_t *pk_table; // (N/2+1)^3 bin ids, 0 = excluded
+    const float2 *cross;      // second field of a cross spectrum (real space), or null
+    const float2 *f2, *f4;    // explicit delta_2, delta_4 (multipoles), or null for L_l(mu)
+    const int16_t *shell_table; // d2 -> shell id, 0 = none, size max_d2+1
+    int max_d2;
+    int nwg;                  // work-groups of the binning kernel
+};
+size_t bin_partials_floats(const BinPlan &b);
+void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStream_t s);
+// Reduces the per-work-group partials in FP64.  pk_out: (5, nbins) Fortran, shell_out: nshell.
+void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, double *pk_out,
+                         double *shell_sum, hipStream_t s);
+
+struct FilterPlan {
+    int N, Np;        // delta grid, maps grid
+    int R;            // max |component| of shell modes
+    int nshell, nell; // shells, 1 or 3
+    float los[3];
+    const int16_t *shell_table;
+    int max_d2;
+    const float2 *f2, *f4;    // explicit delta_2, delta_4 fields, or null for L_l(mu) delta
+};
+void launch_filter(const FilterPlan &f, const float2 *delta, const double *shell_sum,
+                   const double *nmodes, float2 *spectra, float *scale_exp, hipStream_t s);
+
+// ---------------------------------------------------------------- triangles (triangle.hip)
+struct GemmPlan {
+    int nb, nell, ncol, npair;
+    int64_t K, Kstride; // map length and per-map stride (floats)
+    int precision;
+    int kchunk;         // x-values per work-group
+    int nkchunks, nrowblocks;
+};
+GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision);
+size_t gemm_partials_floats(const GemmPlan &g);
+void launch_gemm(const GemmPlan &g, const float *maps, float *partials, hipStream_t s,
+                 KernelTimer *timer);
+void launch_gemm_reduce(const GemmPlan &g, const float *partials, double *C, hipStream_t s);
+
+void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C, int npair,
+                        int ncol, int nb, const float *scale_exp, const double *shell_sum,
+                        const double *nmodes, const float *counts, double kf3, double powSN,
+                        double gridD, double sum_scale, double kcenter, double kbin, double *bout,
+                        hipStream_t s);
+
+void count_pairs_gpu(int N, int nb, const int32_t *modes_d, int64_t nmodes, const int64_t *shell_off,
+                     const int32_t *canon_d, const int32_t *canon_w, int64_t ncanon,
+                     const int16_t *shell_table_d, int max_d2, unsigned long long *T_d,
+                     hipStream_t s);
+
+// ---------------------------------------------------------------- timing
+// HIP events around one named kernel on the stream it is launched on (benchmark support).
+struct KernelTimer {
+    std::string name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+struct TimedScope {
+    KernelTimer *t;
+    hipStream_t s;
+    hipEvent_t b = nullptr;
+    TimedScope(KernelTimer *t_, const char *k, hipStream_t s_);
+    ~TimedScope();
+};
+
+void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
+
+}  // namespace pbi4

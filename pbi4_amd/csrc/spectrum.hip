@@ -1,0 +1,364 @@
+// Fourier-space kernels: interlacing combine (fcomb), P_l / shell-power binning, shell filter.
+#include "common.h"
+
+namespace pbi4 {
+
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// ------------------------------------------------------------------ fcomb (powerI4.f90:251-378)
+
+// F(X,Y,Z) = G1 + i G2 of the reference's c2c transform of (grid1 + i grid2), read from the two
+// r2c half spectra (H = N/2+1 entries along x): G(k) = conj(G(-k)) for X > N/2.
+__device__ __forceinline__ float2 read_F(int N, int H, const float2 *G1, const float2 *G2, int X,
+                                         int Y, int Z) {
+    float2 g1, g2;
+    if (X <= N / 2) {
+        const int64_t o = ((int64_t)Z * N + Y) * H + X;
+        g1 = G1[o];
+        g2 = G2[o];
+    } else {
+        const int64_t o = ((int64_t)((N - Z) % N) * N + (N - Y) % N) * H + (N - X);
+        g1 = conjf2(G1[o]);
+        g2 = conjf2(G2[o]);
+    }
+    return make_float2(g1.x - g2.y, g1.y + g2.x);
+}
+
+// One thread per iteration (ix, iy, iz) in [0, N/2]^3 of the reference's interlaced loop.  The
+// iteration owns the <= 8 cells {ix,icx} x {iy,icy} x {iz,icz}; their eight ordered stores are
+// replayed in registers (a later store to the same cell wins, later stores read earlier ones),
+// then the cells in the x <= N/2 half are written.  Code bits: 1 = icx, 2 = icy, 4 = icz.
+__global__ void k_combine_interlaced(int N, float theta, const float *wtab, float cf,
+                                     const float2 *G1, const float2 *G2, float2 *delta) {
+    const int h = N / 2 + 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)h * h * h) return;
+    const int ix = (int)(t % h), iy = (int)((t / h) % h), iz = (int)(t / ((int64_t)h * h));
+    const int icx = (N - ix) % N, icy = (N - iy) % N, icz = (N - iz) % N;
+    const int mask = (ix == icx ? 1 : 0) | (iy == icy ? 2 : 0) | (iz == icz ? 4 : 0);
+
+    float2 orig[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        orig[k] = read_F(N, h, G1, G2, (k & 1) ? icx : ix, (k & 2) ? icy : iy, (k & 4) ? icz : iz);
+
+    auto cm = [&](int s) {  // i * exp(-i theta s)
+        float sn, cs;
+        sincosf(-theta * (float)s, &sn, &cs);
+        return make_float2(-sn, cs);
+    };
+    auto comb = [&](float2 a, float2 b, float2 c) {  // a (1 - c) + conj(b) (1 + c)
+        return cadd(cmul(a, make_float2(1.0f - c.x, -c.y)), cmul(conjf2(b), make_float2(1.0f + c.x, c.y)));
+    };
+    const float cfac = cf / (wtab[ix] * wtab[iy] * wtab[iz]);
+    const float2 c000 = cscale(comb(orig[0], orig[7], cm(ix + iy + iz)), cfac);
+    const float2 c001 = cscale(comb(orig[4], orig[3], cm(ix + iy - iz)), cfac);
+    const float2 c010 = cscale(comb(orig[2], orig[5], cm(ix - iy + iz)), cfac);
+    const float2 c011 = cscale(comb(orig[6], orig[1], cm(ix - iy - iz)), cfac);
+
+    float2 mem[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mem[k] = make_float2(0.0f, 0.0f);
+    auto put = [&](int code, float2 v) {
+        const int cc = code & ~mask;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j == cc) mem[j] = v;
+    };
+    auto get = [&](int code) {
+        const int cc = code & ~mask;
+        float2 v = mem[0];
+#pragma unroll
+        for (int j = 1; j < 8; ++j)
+            if (j == cc) v = mem[j];
+        return v;
+    };
+    put(0, c000);
+    put(4, c001);
+    put(2, c010);
+    put(6, c011);
+    put(1, conjf2(get(6)));
+    put(5, conjf2(get(2)));
+    put(3, conjf2(get(4)));
+    put(7, conjf2(get(0)));
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) {
+        const int Y = (k & 2) ? icy : iy, Z = (k & 4) ? icz : iz;
+        delta[((int64_t)Z * N + Y) * h + ix] = get(k);
+    }
+}
+
+// Non-interlaced fcomb: only the window deconvolution and the 1/kF^3 normalisation of the
+// ix <= N/2 half (powerI4.f90:351-374).
+__global__ void k_combine_single(int N, const float *wtab, float cf, const float2 *G1,
+                                 float2 *delta) {
+    const int h = N / 2 + 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * N * h) return;
+    const int x = (int)(t % h), y = (int)((t / h) % N), z = (int)(t / ((int64_t)h * N));
+    const int ay = y <= N / 2 ? y : N - y, az = z <= N / 2 ? z : N - z;
+    delta[t] = cscale(G1[t], cf / (wtab[x] * wtab[ay] * wtab[az]));
+}
+
+// ------------------------------------------------------------------ binning
+
+constexpr int kBinThreads = 256;
+constexpr int kBinWaves = kBinThreads / 64;
+
+// Segmented suffix sum across a wave for runs of equal key (keys non-decreasing in lane order);
+// afterwards the first lane of each run holds the run total.
+template <int NV>
+__device__ __forceinline__ void seg_suffix_sum(int key, float (&v)[NV], int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int k2 = __shfl_down(key, off);
+        float t[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) t[i] = __shfl_down(v[i], off);
+        if (lane + off < 64 && k2 == key) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v[i] += t[i];
+        }
+    }
+}
+
+// One wave per (y, z) row of the half spectrum, 64 consecutive x per step.  Along a row |k|
+// grows with x, so P(k) bins and shell ids are non-decreasing in lane order and each run is
+// reduced in registers; the run heads add into wave-private LDS accumulators (no atomics), the
+// waves are summed in a fixed order and each work-group writes one partial: deterministic.
+__global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *delta,
+                                                     float *partials) {
+    extern __shared__ float acc[];
+    const int N = b.N, h = N / 2 + 1;
+    const int nq = 5 * b.nbins_pk + b.nshell;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < kBinWaves * nq; i += blockDim.x) acc[i] = 0.0f;
+    __syncthreads();
+    float *mine = acc + wave * nq;
+    const int64_t nrows = (int64_t)N * N;
+    for (int64_t row = (int64_t)blockIdx.x * kBinWaves + wave; row < nrows;
+         row += (int64_t)gridDim.x * kBinWaves) {
+        const int y = (int)(row % N), z = (int)(row / N);
+        const int ky = y <= N / 2 ? y : y - N, kz = z <= N / 2 ? z : z - N;
+        const int aky = ky < 0 ? -ky : ky, akz = kz < 0 ? -kz : kz;
+        const float lt = b.los[1] * (float)ky + b.los[2] * (float)kz;
+        for (int x0 = 0; x0 < h; x0 += 64) {
+            const int x = x0 + lane;
+            const bool valid = x < h;
+            float2 d = make_float2(0.0f, 0.0f);
+            if (valid) d = delta[row * h + x];
+            const float p = d.x * d.x + d.y * d.y;
+            auto dot = [&](const float2 *f) {   // Re(conj(d) * f) at this mode
+                const float2 e = f[row * h + x];
+                return d.x * e.x + d.y * e.y;
+            };
+            if (b.nbins_pk > 0) {
+                int key = -1;
+                float v[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                if (valid) {
+                    const bool sel = x != 0 || ky > 0 || (ky == 0 && kz > 0);
+                    const int bin = sel ? (int)b.pk_table[((int64_t)x * h + aky) * h + akz] : 0;
+                    key = bin;
+                    if (bin > 0) {
+                        const float k2 = (float)(x * x + aky * aky + akz * akz);
+                        const float lk = b.los[0] * (float)x + lt;
+                        const float mu2 = k2 > 0.0f ? lk * lk / k2 : 0.0f;
+                        v[0] = 1.0f;
+                        v[1] = b.kF * sqrtf(k2);
+                        v[2] = (!b.multipoles && b.cross) ? dot(b.cross) : p;
+                        if (b.multipoles) {
+                            v[3] = b.f2 ? dot(b.f2) : (-0.5f + 1.5f * mu2) * p;
+                            v[4] = b.f4 ? dot(b.f4) : (0.375f + mu2 * (-3.75f + 4.375f * mu2)) * p;
+                        }
+                    }
+                }
+                seg_suffix_sum<5>(key, v, lane);
+                const int prev = __shfl_up(key, 1);
+                if (key > 0 && (lane == 0 || prev != key)) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) mine[q * b.nbins_pk + key - 1] += v[q];
+                }
+            }
+            if (b.nshell > 0) {
+                int key = -1;
+                float v[1] = {0.0f};
+                if (valid) {
+                    const int d2 = x * x + aky * aky + akz * akz;
+                    const int s = d2 <= b.max_d2 ? (int)b.shell_table[d2] : 0;
+                    key = s;
+                    if (s > 0) {
+                        const bool planex = x == 0 || x == N / 2;
+                        const bool special = planex && (y == 0 || y == N / 2) && (z == 0 || z == N / 2);
+                        v[0] = (special ? d.x * d.x : p) * (planex ? 1.0f : 2.0f);
+                    }
+                }
+                seg_suffix_sum<1>(key, v, lane);
+                const int prev = __shfl_up(key, 1);
+                if (key > 0 && (lane == 0 || prev != key)) mine[5 * b.nbins_pk + key - 1] += v[0];
+            }
+        }
+    }
+    __syncthreads();
+    float *out = partials + (int64_t)blockIdx.x * nq;
+    for (int i = threadIdx.x; i < nq; i += blockDim.x) {
+        float s = acc[i];
+        for (int w = 1; w < kBinWaves; ++w) s += acc[w * nq + i];
+        out[i] = s;
+    }
+}
+
+__global__ void k_bin_finalize(BinPlan b, const float *partials, double kF3, double *pk_out,
+                               double *shell_sum) {
+    const int nq = 5 * b.nbins_pk + b.nshell;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < b.nbins_pk) {
+        double s[5] = {0, 0, 0, 0, 0};
+        for (int g = 0; g < b.nwg; ++g)
+            for (int q = 0; q < 5; ++q) s[q] += (double)partials[(int64_t)g * nq + q * b.nbins_pk + i];
+        double *p = pk_out + 5 * (int64_t)i;
+        const double co = s[0];
+        p[0] = s[1] / co;
+        p[1] = s[2] * kF3 / co;
+        if (b.multipoles) {
+            p[2] = 5.0 * s[3] * kF3 / co;
+            p[3] = 9.0 * s[4] * kF3 / co;
+            p[4] = co;
+        } else {
+            p[2] = co;
+            p[3] = 0.0;
+            p[4] = 0.0;
+        }
+    } else if (i < b.nbins_pk + b.nshell) {
+        const int s = i - b.nbins_pk;
+        double t = 0.0;
+        for (int g = 0; g < b.nwg; ++g) t += (double)partials[(int64_t)g * nq + 5 * b.nbins_pk + s];
+        shell_sum[s] = t;
+    }
+}
+
+// ------------------------------------------------------------------ shell filter
+
+// One thread per mode of the maps-grid half spectrum (N' = f.Np).  A mode q inside the shells
+// takes the reference's full-spectrum value, made exactly Hermitian on the self-conjugate x
+// planes (Re of the reference's c2c maps equals the c2r of the Hermitian part), times L_l(mu)
+// for the redshift-space maps and the power-of-two scale 2^e_s of its shell; every other map
+// gets zero at that mode.  Map m = ell*nshell + (s-1).
+__device__ __forceinline__ float2 herm_at(const float2 *f, int N, int x, int Y, int Z) {
+    const int h = N / 2 + 1;
+    const float2 a = f[((int64_t)Z * N + Y) * h + x];
+    if (x == 0 || 2 * x == N) {
+        const float2 m = f[((int64_t)((N - Z) % N) * N + (N - Y) % N) * h + x];
+        return make_float2(0.5f * (a.x + m.x), 0.5f * (a.y - m.y));
+    }
+    return a;
+}
+
+__global__ void k_filter(FilterPlan f, const float2 *delta, const float *scale_exp,
+                         float2 *spectra) {
+    const int Np = f.Np, hp = Np / 2 + 1;
+    const int64_t per_map = (int64_t)Np * Np * hp;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per_map) return;
+    const int x = (int)(t % hp), y = (int)((t / hp) % Np), z = (int)(t / ((int64_t)hp * Np));
+    const int qy = y <= Np / 2 ? y : y - Np, qz = z <= Np / 2 ? z : z - Np;
+    int s = 0;
+    float2 H = make_float2(0.0f, 0.0f), H2 = H, H4 = H;
+    const int R = f.R;
+    if (x <= R && qy <= R && qy >= -R && qz <= R && qz >= -R) {
+        const int d2 = x * x + qy * qy + qz * qz;
+        s = d2 <= f.max_d2 ? f.shell_table[d2] : 0;
+        if (s > f.nshell) s = 0;
+        if (s > 0) {
+            const int N = f.N;
+            const int Y = (qy + N) % N, Z = (qz + N) % N;
+            const float sc = ldexpf(1.0f, (int)scale_exp[s - 1]);
+            H = cscale(herm_at(delta, N, x, Y, Z), sc);
+            if (f.nell == 3) {
+                const float lk = f.los[0] * x + f.los[1] * qy + f.los[2] * qz;
+                const float mu2 = d2 > 0 ? lk * lk / (float)d2 : 0.0f;
+                H2 = f.f2 ? cscale(herm_at(f.f2, N, x, Y, Z), sc) : cscale(H, -0.5f + 1.5f * mu2);
+                H4 = f.f4 ? cscale(herm_at(f.f4, N, x, Y, Z), sc)
+                          : cscale(H, 0.375f + mu2 * (-3.75f + 4.375f * mu2));
+            }
+        }
+    }
+    const float2 zero = make_float2(0.0f, 0.0f);
+    for (int m = 0; m < f.nshell; ++m) spectra[(int64_t)m * per_map + t] = (m + 1 == s) ? H : zero;
+    if (f.nell == 3) {
+        for (int m = 0; m < f.nshell; ++m)
+            spectra[(int64_t)(f.nshell + m) * per_map + t] = (m + 1 == s) ? H2 : zero;
+        for (int m = 0; m < f.nshell; ++m)
+            spectra[(int64_t)(2 * f.nshell + m) * per_map + t] = (m + 1 == s) ? H4 : zero;
+    }
+}
+
+// Power-of-two shell scales: e_s = -round(log2(sqrt(sum_{q in S_s} |H_q|^2))), so each map has
+// O(1) RMS in FP32 / BF16 whatever the catalogue normalisation.
+__global__ void k_shell_scales(int nshell, const double *shell_sum, float *scale_exp) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshell) return;
+    const double v = shell_sum[s];
+    float e = 0.0f;
+    if (v > 0.0 && isfinite(v)) e = (float)(-rint(0.5 * log2(v)));
+    scale_exp[s] = e;
+}
+
+}  // namespace
+
+void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
+                    const float2 *G2, float2 *delta, hipStream_t s) {
+    const int h = N / 2 + 1;
+    if (interlacing) {
+        const float theta = (float)((double)kTwoPiF32 / 2.0 / (double)N);
+        const int64_t n = (int64_t)h * h * h;
+        hipLaunchKernelGGL(k_combine_interlaced, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                           N, theta, wtab, cf, G1, G2, delta);
+    } else {
+        const int64_t n = (int64_t)N * N * h;
+        hipLaunchKernelGGL(k_combine_single, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, N,
+                           wtab, cf, G1, delta);
+    }
+    PBI4_HIP(hipGetLastError());
+}
+
+size_t bin_partials_floats(const BinPlan &b) {
+    return (size_t)b.nwg * (5 * b.nbins_pk + b.nshell);
+}
+
+void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStream_t s) {
+    const size_t lds = sizeof(float) * kBinWaves * (5 * b.nbins_pk + b.nshell);
+    PBI4_CHECK(lds <= 160 * 1024, PBI4_EUNSUPPORTED, "too many bins for the binning kernel");
+    if (lds > 64 * 1024)
+        PBI4_HIP(hipFuncSetAttribute((const void *)k_bin, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+    hipLaunchKernelGGL(k_bin, dim3(b.nwg), dim3(kBinThreads), lds, s, b, delta, partials);
+    PBI4_HIP(hipGetLastError());
+}
+
+void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, double *pk_out,
+                         double *shell_sum, hipStream_t s) {
+    const int n = b.nbins_pk + b.nshell;
+    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, 64)), dim3(64), 0, s, b, partials,
+                       kF3, pk_out, shell_sum);
+    PBI4_HIP(hipGetLastError());
+}
+
+void launch_filter(const FilterPlan &f, const float2 *delta, const double *shell_sum,
+                   const double *nmodes, float2 *spectra, float *scale_exp, hipStream_t s) {
+    (void)nmodes;
+    hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)ceil_div(f.nshell, 64)), dim3(64), 0, s,
+                       f.nshell, shell_sum, scale_exp);
+    const int64_t per_map = (int64_t)f.Np * f.Np * (f.Np / 2 + 1);
+    hipLaunchKernelGGL(k_filter, dim3((unsigned)ceil_div(per_map, 256)), dim3(256), 0, s, f, delta,
+                       scale_exp, spectra);
+    PBI4_HIP(hipGetLastError());
+}
+
+}  // namespace pbi4

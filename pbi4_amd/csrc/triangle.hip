@@ -1,0 +1,340 @@
+// Bispectrum triangle sums as one MFMA GEMM, FP64 finalisation, exact triangle counts.
+//
+// C[p][c] = sum_x A[p][x] B[x][c] with A[(i<=j)][x] = M_i(x) M_j(x) (formed on the fly from the
+// ell = 0 shell maps staged in LDS) and B[x][(ell,l)] = M_{ell,l}(x) -- the triple sums
+// sum_x M_i M_j M_{ell,l} of bispectrumI4.f90:164-193 and :476-511 for every (i<=j, ell, l).
+// Split-K: each work-group owns 64 pair-rows x all columns x one K chunk and writes an FP32
+// partial; the partials are summed over chunks in FP64 in a fixed order.
+//   PBI4_PREC_FP32  : v_mfma_f32_16x16x4_f32 (the plain path).
+//   PBI4_PREC_BF16X3: a = a_hi + a_lo, b = b_hi + b_lo in bf16 (round-to-nearest), and
+//                     a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi on v_mfma_f32_16x16x32_bf16 with
+//                     FP32 accumulation (relative representation error ~2^-17 per product).
+#include "common.h"
+
+namespace pbi4 {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGemmThreads = 256;     // 4 waves, 16 pair-rows each
+constexpr int kRowsPerWG = 64;
+constexpr int kKt = 64;               // x-values staged per LDS tile
+constexpr int kPadF = 4;              // float padding per LDS row
+constexpr int kPadB = 8;              // bf16 padding per LDS row
+constexpr int kMaxCF = 12;            // column fragments (ncol <= 192)
+
+__device__ __forceinline__ void pair_of(int p, int &i, int &j) {
+    // p = j(j+1)/2 + i with 0 <= i <= j
+    int jj = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+    while ((jj + 1) * (jj + 2) / 2 <= p) ++jj;
+    while (jj * (jj + 1) / 2 > p) --jj;
+    j = jj;
+    i = p - jj * (jj + 1) / 2;
+}
+
+template <int PREC, int CF>
+__global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *maps, int64_t K,
+                                                                int64_t Kstride, int nb, int ncol,
+                                                                int npair, int kchunk,
+                                                                float *partials) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ldf = kKt + kPadF;
+    float *tf = reinterpret_cast<float *>(smem);                          // [ncol][ldf]
+    __bf16 *bhi = reinterpret_cast<__bf16 *>(smem + sizeof(float) * ncol * ldf);
+    const int ldb = kKt + kPadB;
+    __bf16 *blo = bhi + CF * 16 * ldb;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rowblock = blockIdx.x, chunk = blockIdx.y;
+    const int64_t kbeg = (int64_t)chunk * kchunk;
+    const int64_t kend = min(K, kbeg + kchunk);
+
+    // this lane's pair-row for the A operand
+    const int prow = rowblock * kRowsPerWG + wave * 16 + (lane & 15);
+    int pi = 0, pj = 0;
+    if (prow < npair) pair_of(prow, pi, pj);
+    const int grp = lane >> 4;
+
+    f32x4 acc[CF];
+#pragma unroll
+    for (int c = 0; c < CF; ++c) acc[c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    if (PREC == PBI4_PREC_BF16X3) {
+        // padded columns stay zero for the whole kernel
+        for (int e = threadIdx.x; e < (CF * 16 - ncol) * ldb; e += blockDim.x) {
+            const int c = ncol + e / ldb, x = e % ldb;
+            bhi[c * ldb + x] = (__bf16)0.0f;
+            blo[c * ldb + x] = (__bf16)0.0f;
+        }
+    }
+
+    for (int64_t x0 = kbeg; x0 < kend; x0 += kKt) {
+        __syncthreads();
+        // stage ncol x kKt floats (zero beyond K); K is a multiple of 4
+        const int nvec = ncol * (kKt / 4);
+        for (int e = threadIdx.x; e < nvec; e += blockDim.x) {
+            const int c = e / (kKt / 4), v = e % (kKt / 4);
+            const int64_t x = x0 + 4 * v;
+            float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (x < kend) val = *reinterpret_cast<const float4 *>(maps + (int64_t)c * Kstride + x);
+            *reinterpret_cast<float4 *>(tf + c * ldf + 4 * v) = val;
+        }
+        __syncthreads();
+        if (PREC == PBI4_PREC_BF16X3) {
+            for (int e = threadIdx.x; e < ncol * kKt; e += blockDim.x) {
+                const int c = e / kKt, x = e % kKt;
+                const float v = tf[c * ldf + x];
+                const __bf16 h = (__bf16)v;
+                bhi[c * ldb + x] = h;
+                blo[c * ldb + x] = (__bf16)(v - (float)h);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < kKt; ks += 32) {
+                const int xo = ks + 8 * grp;
+                const float4 *ai = reinterpret_cast<const float4 *>(tf + pi * ldf + xo);
+                const float4 *aj = reinterpret_cast<const float4 *>(tf + pj * ldf + xo);
+                const float4 a0 = ai[0], a1 = ai[1], b0 = aj[0], b1 = aj[1];
+                const float prod[8] = {a0.x * b0.x, a0.y * b0.y, a0.z * b0.z, a0.w * b0.w,
+                                       a1.x * b1.x, a1.y * b1.y, a1.z * b1.z, a1.w * b1.w};
+                bf16x8 ahi, alo;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const __bf16 h = (__bf16)prod[e];
+                    ahi[e] = h;
+                    alo[e] = (__bf16)(prod[e] - (float)h);
+                }
+#pragma unroll
+                for (int c = 0; c < CF; ++c) {
+                    const int col = c * 16 + (lane & 15);
+                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(bhi + col * ldb + xo);
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(blo + col * ldb + xo);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bh, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bh, acc[c], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < kKt; ks += 16) {
+                const int xo = ks + 4 * grp;
+                const float4 a = *reinterpret_cast<const float4 *>(tf + pi * ldf + xo);
+                const float4 b = *reinterpret_cast<const float4 *>(tf + pj * ldf + xo);
+                const float pr[4] = {a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w};
+#pragma unroll
+                for (int c = 0; c < CF; ++c) {
+                    const int col = c * 16 + (lane & 15);
+                    float4 bv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (col < ncol) bv = *reinterpret_cast<const float4 *>(tf + col * ldf + xo);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[0], bv.x, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[1], bv.y, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[2], bv.z, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[3], bv.w, acc[c], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D layout of the 16x16 MFMA: lane holds rows 4*(lane/16) + r, column lane%16
+    const int ncol_pad = CF * 16;
+    const int64_t npair_pad = (int64_t)gridDim.x * kRowsPerWG;
+    float *out = partials + (int64_t)chunk * npair_pad * ncol_pad;
+#pragma unroll
+    for (int c = 0; c < CF; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = (int64_t)rowblock * kRowsPerWG + wave * 16 + 4 * grp + r;
+            out[row * ncol_pad + c * 16 + (lane & 15)] = acc[c][r];
+        }
+}
+
+__global__ void k_gemm_reduce(const float *partials, int nchunks, int64_t npair_pad, int ncol_pad,
+                              int npair, int ncol, double *C) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)npair * ncol) return;
+    const int64_t p = t / ncol, c = t % ncol;
+    double s = 0.0;
+    for (int k = 0; k < nchunks; ++k) s += (double)partials[((int64_t)k * npair_pad + p) * ncol_pad + c];
+    C[t] = s;
+}
+
+// One thread per triangle m (reference order): bout(:, m) of bispectrumI4.f90:173-182 / :489-500
+// with pow(s) from Parseval and the triple sums unscaled from the band-limited maps grid.
+__global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C,
+                              int ncol, int nb, const float *scale_exp, const double *shell_sum,
+                              const double *nmodes, const float *counts, double kf3, double powSN,
+                              double gridD, double sum_scale, double kcenter, double kbin,
+                              double *bout) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= itot) return;
+    const int l = tri[3 * m], j = tri[3 * m + 1], i = tri[3 * m + 2];   // 1-based
+    auto pw = [&](int s) { return kf3 * shell_sum[s - 1] / nmodes[s - 1] - powSN; };
+    const double pl = pw(l), pj = pw(j), pi_ = pw(i);
+    const double powSN2 = powSN * powSN;
+    const double bispSN = ((pi_ + pj) + pl) * powSN + powSN2;
+    const int64_t p = (int64_t)(j - 1) * j / 2 + (i - 1);
+    const double unscale =
+        sum_scale * ldexp(1.0, -(int)(scale_exp[i - 1] + scale_exp[j - 1] + scale_exp[l - 1]));
+    const double cnt = (double)counts[m];
+    double *b = bout + (int64_t)rows * m;
+    const double sum = C[p * ncol + (l - 1)] * unscale;
+    b[0] = kcenter + (double)(l - 1) * kbin;
+    b[1] = kcenter + (double)(j - 1) * kbin;
+    b[2] = kcenter + (double)(i - 1) * kbin;
+    b[3] = pl;
+    b[4] = pj;
+    b[5] = pi_;
+    b[6] = kf3 * sum / cnt + bispSN;
+    b[7] = bispSN;
+    b[8] = (double)(float)(cnt / gridD);
+    if (rows == 11) {
+        const double s2 = C[p * ncol + nb + (l - 1)] * unscale;
+        const double s4 = C[p * ncol + 2 * nb + (l - 1)] * unscale;
+        b[9] = 5.0 * kf3 * s2 / cnt;
+        b[10] = 9.0 * kf3 * s4 / cnt;
+    }
+}
+
+// ------------------------------------------------------------------ exact triangle counts
+// One work-group per canonical q1 (|q1x| >= |q1y| >= |q1z| >= 0 up to the lattice symmetry
+// group, weighted by its orbit size): for every q2 in shells j >= shell(q1), the shell l of
+// q3 = -(q1 + q2) mod N is histogrammed in LDS; the histogram times the weight is added to
+// T[i][j][l] (uint64).  Integer arithmetic only: the counts are exact.
+constexpr int kCountThreads = 256;
+
+__global__ __launch_bounds__(kCountThreads) void k_count_pairs(int N, int nb,
+                                                               const int32_t *modes,
+                                                               const int64_t *shell_off,
+                                                               const int32_t *canon,
+                                                               const int32_t *canon_w,
+                                                               const int16_t *shell_table,
+                                                               int max_d2,
+                                                               unsigned long long *T) {
+    extern __shared__ uint32_t hist[];   // (nb+1) x (nb+1): [j][l]
+    const int S = nb + 1;
+    for (int e = threadIdx.x; e < S * S; e += blockDim.x) hist[e] = 0;
+    __syncthreads();
+    const int qx = canon[4 * blockIdx.x], qy = canon[4 * blockIdx.x + 1];
+    const int qz = canon[4 * blockIdx.x + 2], si = canon[4 * blockIdx.x + 3];
+    const int64_t b = shell_off[si - 1], e = shell_off[nb];
+    for (int64_t q = b + threadIdx.x; q < e; q += blockDim.x) {
+        const int mx = modes[4 * q], my = modes[4 * q + 1], mz = modes[4 * q + 2];
+        const int sj = modes[4 * q + 3];
+        int cx = -(qx + mx), cy = -(qy + my), cz = -(qz + mz);
+        cx = ((cx % N) + N) % N;
+        cy = ((cy % N) + N) % N;
+        cz = ((cz % N) + N) % N;
+        const int dx = min(cx, N - cx), dy = min(cy, N - cy), dz = min(cz, N - cz);
+        const int d2 = dx * dx + dy * dy + dz * dz;
+        const int sl = d2 <= max_d2 ? shell_table[d2] : 0;
+        if (sl > 0 && sl <= nb) atomicAdd(&hist[sj * S + sl], 1u);
+    }
+    __syncthreads();
+    const unsigned long long w = (unsigned long long)canon_w[blockIdx.x];
+    for (int t = threadIdx.x; t < S * S; t += blockDim.x) {
+        const uint32_t h = hist[t];
+        if (h) atomicAdd(&T[(int64_t)si * S * S + t], w * h);
+    }
+}
+
+}  // namespace
+
+GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision) {
+    GemmPlan g{};
+    g.nb = nb;
+    g.nell = nell;
+    g.ncol = nb * nell;
+    g.npair = nb * (nb + 1) / 2;
+    g.K = K;
+    g.Kstride = Kstride;
+    g.precision = precision;
+    PBI4_CHECK(g.ncol <= kMaxCF * 16, PBI4_EUNSUPPORTED,
+               "too many shells x multipoles for the triangle GEMM (max 192 columns)");
+    PBI4_CHECK(K % 4 == 0 && Kstride % 4 == 0, PBI4_EINVAL, "maps length must be a multiple of 4");
+    g.nrowblocks = (int)ceil_div(g.npair, kRowsPerWG);
+    // aim for ~2048 work-groups in total, K chunks a multiple of the LDS tile
+    int64_t target = std::max<int64_t>(1, 2048 / g.nrowblocks);
+    int64_t kc = ceil_div(K, target);
+    kc = std::max<int64_t>(kKt * 4, align_up(kc, kKt));
+    g.kchunk = (int)kc;
+    g.nkchunks = (int)ceil_div(K, kc);
+    return g;
+}
+
+size_t gemm_partials_floats(const GemmPlan &g) {
+    const int cf = (int)ceil_div(g.ncol, 16);
+    return (size_t)g.nkchunks * g.nrowblocks * kRowsPerWG * cf * 16;
+}
+
+void launch_gemm(const GemmPlan &g, const float *maps, float *partials, hipStream_t s,
+                 KernelTimer *timer) {
+    const int cf = (int)ceil_div(g.ncol, 16);
+    size_t lds = sizeof(float) * g.ncol * (kKt + kPadF);
+    if (g.precision == PBI4_PREC_BF16X3) lds += 2 * sizeof(__bf16) * cf * 16 * (kKt + kPadB);
+    dim3 grid(g.nrowblocks, g.nkchunks);
+    TimedScope ts(timer, "pbi4_triangle_gemm", s);
+#define PBI4_GEMM_CASE(P, C)                                                                 \
+    case C:                                                                                  \
+        hipLaunchKernelGGL((k_triangle_gemm<P, C>), grid, dim3(kGemmThreads), lds, s, maps, g.K, \
+                           g.Kstride, g.nb, g.ncol, g.npair, g.kchunk, partials);            \
+        break;
+    if (g.precision == PBI4_PREC_BF16X3) {
+        switch (cf) {
+            PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 1) PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 2)
+            PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 3) PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 4)
+            PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 5) PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 6)
+            PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 7) PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 8)
+            PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 9) PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 10)
+            PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 11) PBI4_GEMM_CASE(PBI4_PREC_BF16X3, 12)
+            default: fail(PBI4_EUNSUPPORTED, "bad column fragment count");
+        }
+    } else {
+        switch (cf) {
+            PBI4_GEMM_CASE(PBI4_PREC_FP32, 1) PBI4_GEMM_CASE(PBI4_PREC_FP32, 2)
+            PBI4_GEMM_CASE(PBI4_PREC_FP32, 3) PBI4_GEMM_CASE(PBI4_PREC_FP32, 4)
+            PBI4_GEMM_CASE(PBI4_PREC_FP32, 5) PBI4_GEMM_CASE(PBI4_PREC_FP32, 6)
+            PBI4_GEMM_CASE(PBI4_PREC_FP32, 7) PBI4_GEMM_CASE(PBI4_PREC_FP32, 8)
+            PBI4_GEMM_CASE(PBI4_PREC_FP32, 9) PBI4_GEMM_CASE(PBI4_PREC_FP32, 10)
+            PBI4_GEMM_CASE(PBI4_PREC_FP32, 11) PBI4_GEMM_CASE(PBI4_PREC_FP32, 12)
+            default: fail(PBI4_EUNSUPPORTED, "bad column fragment count");
+        }
+    }
+#undef PBI4_GEMM_CASE
+    PBI4_HIP(hipGetLastError());
+}
+
+void launch_gemm_reduce(const GemmPlan &g, const float *partials, double *C, hipStream_t s) {
+    const int cf = (int)ceil_div(g.ncol, 16);
+    const int64_t n = (int64_t)g.npair * g.ncol;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, partials,
+                       g.nkchunks, (int64_t)g.nrowblocks * kRowsPerWG, cf * 16, g.npair, g.ncol, C);
+    PBI4_HIP(hipGetLastError());
+}
+
+void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C, int npair,
+                        int ncol, int nb, const float *scale_exp, const double *shell_sum,
+                        const double *nmodes, const float *counts, double kf3, double powSN,
+                        double gridD, double sum_scale, double kcenter, double kbin, double *bout,
+                        hipStream_t s) {
+    (void)npair;
+    hipLaunchKernelGGL(k_bk_finalize, dim3((unsigned)ceil_div(itot, 128)), dim3(128), 0, s, rows,
+                       itot, tri, C, ncol, nb, scale_exp, shell_sum, nmodes, counts, kf3, powSN,
+                       gridD, sum_scale, kcenter, kbin, bout);
+    PBI4_HIP(hipGetLastError());
+}
+
+void count_pairs_gpu(int N, int nb, const int32_t *modes_d, int64_t nmodes, const int64_t *shell_off,
+                     const int32_t *canon_d, const int32_t *canon_w, int64_t ncanon,
+                     const int16_t *shell_table_d, int max_d2, unsigned long long *T_d,
+                     hipStream_t s) {
+    (void)nmodes;
+    const size_t lds = sizeof(uint32_t) * (nb + 1) * (nb + 1);
+    PBI4_CHECK(lds <= 64 * 1024, PBI4_EUNSUPPORTED, "too many shells for the count kernel");
+    if (ncanon > 0)
+        hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)ncanon), dim3(kCountThreads), lds, s, N, nb,
+                           modes_d, shell_off, canon_d, canon_w, shell_table_d, max_d2, T_d);
+    PBI4_HIP(hipGetLastError());
+}
+
+}  // namespace pbi4

@@ -1,0 +1,235 @@
+"""GPU parity of each stage of the HIP path against the FP64 oracle and the golden fixtures.
+
+Tolerances (FP32 device arithmetic against FP64):
+  grids: 2e-6 of the largest cell; delta: 2e-5 of max|delta|; P_l: 1e-4 relative of P0 per bin;
+  B: 1e-4 relative with an absolute floor of 1e-4 x the bin's Gaussian scale (see _bk_close);
+  mode counts and N_tr: bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import pbi4_oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pbi4_amd.core import Context
+    return Context(0)
+
+
+def _pos(n, seed, clustered=True):
+    rng = np.random.default_rng(seed)
+    if not clustered:
+        return rng.random((3, n)).astype(np.float32)
+    c = rng.random((3, 6))
+    k = rng.integers(0, 6, n)
+    p = (c[:, k] + 0.06 * rng.standard_normal((3, n))) % 1.0
+    return np.minimum(p.astype(np.float32), np.float32(1.0 - 2 ** -24))
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _grid_to_xyz(t):
+    """[z][y][x] device grid -> numpy indexed [x, y, z]."""
+    return t.cpu().numpy().transpose(2, 1, 0)
+
+
+def _oracle_density(pos, N, order, inter, box=1000.0):
+    n = pos.shape[1]
+    if inter:
+        g = O.assign_double_grid(N, order, n, pos.astype(np.float64))
+    else:
+        g = O.assign_single_grid(N, order, n, pos.astype(np.float64)).astype(np.complex128)
+    return g, O.compute_density(n, g, N, order, inter, box)
+
+
+# ------------------------------------------------------------------ assignment
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("inter", [0, 1])
+@pytest.mark.parametrize("N", [16, 32])
+def test_assign(ctx, order, inter, N):
+    pos = _pos(20000, 100 + order + N)
+    g = ctx.assign(_dev(pos), N, order, inter)
+    torch.cuda.synchronize()
+    ref = (O.assign_double_grid if inter else O.assign_single_grid)(N, order, pos.shape[1],
+                                                                     pos.astype(np.float64))
+    mine = _grid_to_xyz(g[0])
+    exp = ref.real if inter else ref
+    scale = np.abs(exp).max()
+    assert np.abs(mine - exp).max() <= 2e-6 * scale
+    if inter:
+        assert np.abs(_grid_to_xyz(g[1]) - ref.imag).max() <= 2e-6 * scale
+    assert np.allclose(g.double().sum(dim=(1, 2, 3)).cpu().numpy(), pos.shape[1], rtol=1e-5)
+
+
+def test_assign_deterministic_and_box_mode(ctx):
+    N = 32
+    rng = np.random.default_rng(7)
+    box = 1000.0
+    x = (rng.random((3, 30000)) * box).astype(np.float32)
+    v = (rng.standard_normal((3, 30000)) * 300).astype(np.float32)
+    rsd = (0.0, 0.0, np.float32(0.011462))
+    a = ctx.assign(_dev(x), N, 4, True, vel=_dev(v), box=box, rsd=rsd, normalised=False)
+    b = ctx.assign(_dev(x), N, 4, True, vel=_dev(v), box=box, rsd=rsd, normalised=False)
+    assert torch.equal(a, b)
+    # reference normalisation: (s + box) % box / box in float32 (pbi4.py:154, 242-243)
+    s = x.astype(np.float64)
+    s[2] += v[2].astype(np.float64) * np.float64(rsd[2])
+    s = s.astype(np.float32)
+    p = ((s + np.float32(box)) % np.float32(box)) / np.float32(box)
+    ref = O.assign_double_grid(N, 4, p.shape[1], p.astype(np.float64))
+    scale = np.abs(ref.real).max()
+    assert np.abs(_grid_to_xyz(a[0]) - ref.real).max() <= 2e-5 * scale
+
+
+# ------------------------------------------------------------------ density
+
+
+@pytest.mark.parametrize("order,inter", [(4, 1), (3, 1), (2, 1), (3, 0), (4, 0)])
+def test_density(ctx, order, inter):
+    N, n = 32, 40000
+    pos = _pos(n, 5 + order)
+    g = ctx.assign(_dev(pos), N, order, inter)
+    delta = ctx.density(g, N, order, inter, 1000.0, n)
+    torch.cuda.synchronize()
+    _, dcl = _oracle_density(pos, N, order, inter)
+    h = N // 2 + 1
+    mine = delta.cpu().numpy().transpose(2, 1, 0)
+    exp = dcl[:h]
+    assert np.abs(mine - exp).max() <= 2e-5 * np.abs(exp).max()
+
+
+# ------------------------------------------------------------------ P(k)
+
+
+@pytest.mark.parametrize("kb,kc", [(1.0, 1.0), (2.0, 2.0), (1.5, 0.5)])
+def test_pk_multipoles(ctx, kb, kc):
+    N, n, box = 32, 40000, 1000.0
+    pos = _pos(n, 31)
+    g = ctx.assign(_dev(pos), N, 4, True)
+    delta = ctx.density(g, N, 4, True, box, n)
+    los = np.array([0.0, 0.0, 1.0])
+    nb = int((N / 2 + kb - kc) // kb)
+    pout = ctx.pk(delta, N, box, kb, kc, nb, True, los).cpu().numpy().T
+    _, dcl = _oracle_density(pos, N, 4, True, box)
+    d2, d4 = O.compute_density_multipoles(dcl, N, box, los)
+    ref = O.measure_pk_multipoles(N, box, kb, kc, nb, dcl, d2, d4)
+    assert np.array_equal(pout[4], ref[4])
+    ok = ref[4] > 0
+    assert np.allclose(pout[0][ok], ref[0][ok], rtol=1e-6)
+    for r in (1, 2, 3):
+        assert np.all(np.abs(pout[r][ok] - ref[r][ok]) <= 1e-4 * np.abs(ref[1][ok])), r
+    # real-space form
+    pr = ctx.pk(delta, N, box, kb, kc, nb, False).cpu().numpy().T
+    rr = O.measure_pk(N, box, kb, kc, nb, dcl, dcl)
+    assert np.array_equal(pr[2], rr[2])
+    assert np.all(np.abs(pr[1][ok] - rr[1][ok]) <= 1e-4 * np.abs(rr[1][ok]))
+
+
+def test_pk_golden_nmodes(ctx, golden_dir):
+    a = np.loadtxt(os.path.join(golden_dir, "ps_N64_kb1_kc1.txt"))
+    N = 64
+    delta = torch.ones((N, N, N // 2 + 1), dtype=torch.complex64, device="cuda")
+    pout = ctx.pk(delta, N, 1000.0, 1.0, 1.0, 32, True).cpu().numpy().T
+    assert np.array_equal(pout[4].astype(np.int64), a[:, 1].astype(np.int64))
+
+
+# ------------------------------------------------------------------ triangle counts
+
+
+def test_counts_golden(ctx, golden_dir):
+    bs = np.loadtxt(os.path.join(golden_dir, "bs_N64_kb1_kc1.txt")).astype(np.int64)
+    ntr = ctx.count_triangles(21, 64, 1.0, 1.0, True, False, 1156)
+    assert np.array_equal(ntr, bs[:, 3])
+
+
+@pytest.mark.parametrize("N,nb,kb,kc,io,cross", [(12, 4, 1.0, 1.0, True, False),
+                                                 (16, 4, 2.0, 2.0, True, False),
+                                                 (24, 8, 1.0, 1.0, False, False),
+                                                 (16, 5, 1.0, 1.0, True, True)])
+def test_counts_vs_oracle(ctx, N, nb, kb, kc, io, cross):
+    itot = O.itot_count(nb, kb, kc, io, cross)
+    ntr = ctx.count_triangles(nb, N, kb, kc, io, cross, itot)
+    assert np.array_equal(ntr, O.count_triangles_exact(nb, N, kb, kc, io, cross))
+
+
+# ------------------------------------------------------------------ triangle sums (GEMM)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 2e-6), (1, 3e-5)])
+@pytest.mark.parametrize("nb,nell,K", [(3, 1, 1000), (5, 3, 4096), (16, 3, 20000), (20, 1, 7000)])
+def test_triangle_sums(ctx, prec, tol, nb, nell, K):
+    rng = np.random.default_rng(nb * 100 + nell)
+    Ks = ((K + 63) // 64) * 64
+    maps = np.zeros((nb * nell, Ks), dtype=np.float32)
+    maps[:, :K] = rng.standard_normal((nb * nell, K)).astype(np.float32)
+    C = ctx.triangle_sums(_dev(maps), nb, nell, K, prec).cpu().numpy()
+    m = maps[:, :K].astype(np.float64)
+    p = 0
+    for j in range(nb):
+        for i in range(j + 1):
+            prod = m[i] * m[j]
+            exp = m @ prod
+            scale = np.sqrt(np.sum(prod[None, :] ** 2 * m ** 2, axis=1))
+            assert np.all(np.abs(C[p] - exp) <= tol * scale * 5 + 1e-12), (i, j)
+            p += 1
+
+
+# ------------------------------------------------------------------ bispectrum end to end
+
+
+def _bk_close(mine, ref, rows, rtol=1e-4):
+    """Rows 0-5, 7, 8: k's, pow, BSN, Ntr; rows 6, 9, 10: B.  B0 is compared relative to
+    max(|B0_ref|, Gaussian scale sqrt(P1 P2 P3 / Ntr)), the natural size of B in that bin;
+    B2 and B4 relative to max(|B_l|, |B0|, Gaussian scale), as P2 and P4 are relative to P0
+    (B_l crosses zero where B0 does not)."""
+    for r in (0, 1, 2, 8):
+        assert np.array_equal(mine[r], ref[r]), r
+    for r in (3, 4, 5, 7):
+        assert np.allclose(mine[r], ref[r], rtol=rtol), r
+    gauss = np.sqrt(np.abs(ref[3] * ref[4] * ref[5]) / np.maximum(ref[8], 1))
+    b0 = np.abs(ref[6] - ref[7])
+    for r in [6] + ([9, 10] if rows == 11 else []):
+        bref = ref[r] - (ref[7] if r == 6 else 0)
+        bmine = mine[r] - (mine[7] if r == 6 else 0)
+        scale = np.maximum(np.maximum(np.abs(bref), gauss), b0)
+        err = np.abs(bmine - bref) / scale
+        assert err.max() <= rtol, (r, err.max(), np.argmax(err))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("multipoles", [False, True])
+@pytest.mark.parametrize("maps_grid", [0, 32])
+def test_bispectrum(ctx, prec, multipoles, maps_grid):
+    N, n, box, nb, kb, kc = 32, 60000, 1000.0, 6, 1.0, 1.0
+    pos = _pos(n, 77)
+    g = ctx.assign(_dev(pos), N, 4, True)
+    delta = ctx.density(g, N, 4, True, box, n)
+    itot = O.itot_count(nb, kb, kc, True)
+    ntr = ctx.count_triangles(nb, N, kb, kc, True, False, itot)
+    counts = (ntr.astype(np.float64) * N ** 3).astype(np.float32)
+    kf = np.float32(2 * np.pi / box)
+    los = (0.0, 0.0, 1.0)
+    desc = ctx.bk_desc(nb, N, kb, kc, True, kf, itot, n, multipoles, los, prec, maps_grid)
+    if maps_grid == 0:
+        assert ctx.maps_grid(desc) < N          # band-limited grid chosen
+    bout = ctx.bk(delta, _dev(counts), desc).cpu().numpy().T
+    _, dcl = _oracle_density(pos, N, 4, True, box)
+    if multipoles:
+        d2, d4 = O.compute_density_multipoles(dcl, N, box, np.array(los))
+        ref = O.measure_bk(nb, N, kb, kc, True, kf, itot, n, counts, dcl, d2, d4)
+    else:
+        ref = O.measure_bk(nb, N, kb, kc, True, kf, itot, n, counts, dcl)
+    _bk_close(bout, ref, 11 if multipoles else 9)
