@@ -1,0 +1,122 @@
+"""One catalogue through the whole measurement path on one HIP stream.
+
+A Measurement describes what a request asks for (the parameters of IO.derived_params,
+pbi4/pbi4.py:86-119, plus the fields to measure); CataloguePlan owns everything that is shared
+between catalogues of that shape (triangle list and exact counts, bispectrum descriptors) and
+enqueues, for every field, assignment -> density -> P_l -> B_l with no host synchronisation.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .catalogue import rsd_factor
+from .core import Context, kf3_f32
+
+
+def itot_count(nbins, kbin, kcenter, iopen, cross=False):
+    """Number of triangles of the reference loop (pbi4/pbi4.py:105-117)."""
+    ishift = int(kcenter / kbin) + (1 if iopen else 0)
+    n = 0
+    for l in range(1, nbins + 1):
+        for j in range(1, l + 1):
+            imax = min(nbins, j + l - 1 + ishift) if cross else j
+            n += max(0, imax - max(1, l - j + 1 - ishift) + 1)
+    return n
+
+
+@dataclass(frozen=True)
+class Field:
+    """A density field to measure: real space, or redshift space along `los` (already divided
+    by |los|^2 as pbi4/pbi4.py:103 does)."""
+    rsd: bool = False
+    los: tuple = (0.0, 0.0, 1.0)
+
+
+@dataclass(frozen=True)
+class Measurement:
+    grid: int = 256
+    order: int = 4
+    interlacing: bool = True
+    box: float = 1000.0
+    kbin: float = 2.0
+    kcenter: float = 2.0
+    nbins_ps: int = 0            # 0 = floor((N/2 + kbin - kcenter)/kbin)
+    nbins_bisp: int = 16         # 0 = floor((N/3 + kbin - kcenter)/kbin)
+    iopen: bool = True
+    fields: tuple = field(default_factory=lambda: (Field(False), Field(True)))
+    precision: int = _lib.PREC_BF16X3
+    maps_grid: int = 0
+    z: float = 0.5
+    om: float = 0.3
+    ol: float = 0.7
+
+    def nb_ps(self):
+        return self.nbins_ps or int((self.grid / 2 + self.kbin - self.kcenter) // self.kbin)
+
+    def nb_bisp(self):
+        return self.nbins_bisp or int((self.grid / 3 + self.kbin - self.kcenter) // self.kbin)
+
+    def itot(self):
+        return itot_count(self.nb_bisp(), self.kbin, self.kcenter, self.iopen)
+
+    def kf(self):
+        return np.float32(2.0 * np.pi / self.box)
+
+
+class CataloguePlan:
+    """Shared state for catalogues of one Measurement shape on one context."""
+
+    def __init__(self, ctx: Context, m: Measurement):
+        self.ctx, self.m = ctx, m
+        nb = m.nb_bisp()
+        self.itot = m.itot()
+        ntr = ctx.count_triangles(nb, m.grid, m.kbin, m.kcenter, m.iopen, False, self.itot)
+        self.ntr = ntr
+        # the reference's counts(m) = N_tr N^3 as REAL(4) (bispectrumI4.f90:8, f2py coercion)
+        counts = (ntr.astype(np.float64) * float(m.grid) ** 3).astype(np.float32)
+        self.counts = torch.from_numpy(counts).to(ctx.device)
+        self.desc = {}
+        for f in m.fields:
+            self.desc[f] = ctx.bk_desc(nb, m.grid, m.kbin, m.kcenter, m.iopen, m.kf(), self.itot,
+                                       1, f.rsd, f.los, m.precision, m.maps_grid)
+        self.maps_grid = ctx.maps_grid(next(iter(self.desc.values())))
+        self.bk_ws_bytes = max(ctx.bk_workspace(d) for d in self.desc.values())
+        self.rsd_coef = rsd_factor(m.z, m.om, m.ol)
+
+    def bytes_per_catalogue(self, npart):
+        """Device memory one catalogue of npart particles needs while in flight (workspace
+        estimate used for admission)."""
+        N = self.m.grid
+        V = N ** 3
+        h = N * N * (N // 2 + 1)
+        assign_ws = 12 * npart + 4 * 3 * (V // 64 + 1) + 8 * V * 2 * 2   # sort + regions (<= 2x)
+        grids = 8 * V
+        dens = 2 * 8 * h + 8 * h
+        return int(assign_ws + grids + dens + self.bk_ws_bytes + 8 * h) + (64 << 20)
+
+    def run(self, pos, vel, npart, stream=None, precision=None):
+        """Enqueue every field of one catalogue; returns {field: (pout, bout)} device tensors.
+        pos/vel: (3, npart) float32 box-unit positions and velocities (vel may be None when no
+        field is in redshift space)."""
+        m, ctx = self.m, self.ctx
+        out = {}
+        s = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            for f in m.fields:
+                rsd = tuple(float(np.float32(self.rsd_coef * l)) for l in f.los) if f.rsd else None
+                grids = ctx.assign(pos, m.grid, m.order, m.interlacing, vel=vel if f.rsd else None,
+                                   box=m.box, rsd=rsd, normalised=False, stream=s)
+                delta = ctx.density(grids, m.grid, m.order, m.interlacing, m.box, npart, stream=s)
+                del grids
+                pout = ctx.pk(delta, m.grid, m.box, m.kbin, m.kcenter, m.nb_ps(), f.rsd, f.los,
+                              stream=s)
+                desc = self.desc[f]
+                desc.npart = int(npart)
+                if precision is not None:
+                    desc.precision = int(precision)
+                bout = ctx.bk(delta, self.counts, desc, stream=s)
+                desc.precision = int(m.precision)
+                out[f] = (pout, bout)
+        return out

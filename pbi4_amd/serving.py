@@ -1,0 +1,157 @@
+"""Device-resident serving loop for covariance-style workloads (continuous batching).
+
+Requests (one catalogue each, any grid / order / particle count) queue in arrival order.  A
+request is admitted when a slot is free and the device has room for its in-flight memory
+(estimated per request, checked against hipMemGetInfo plus what torch's caching allocator holds
+unused); each admitted request runs its whole pipeline on its slot's own HIP stream, so requests
+that arrive while others are on the GPU start as soon as a slot and memory free up and their
+kernels overlap on the device.  Plans (rocFFT plans, exact triangle counts, bin and shell
+tables) are built once per request shape and stay resident.  Completion is detected with HIP
+events; results come back through pinned host buffers.  Latency = completion - arrival.
+
+No counterpart in the reference, which measures one catalogue per process run.
+"""
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .pipeline import CataloguePlan, Measurement
+
+
+@dataclass
+class Request:
+    rid: int
+    measurement: Measurement
+    pos: torch.Tensor              # (3, npart) float32 box units, on the device
+    vel: torch.Tensor = None       # (3, npart) float32 km/s, on the device (redshift space)
+    arrival: float = 0.0           # seconds, engine clock
+    admitted: float = None
+    done: float = None
+    result: dict = None
+
+    @property
+    def npart(self):
+        return int(self.pos.shape[1])
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+@dataclass
+class _Slot:
+    stream: torch.cuda.Stream
+    req: Request = None
+    event: torch.cuda.Event = None
+    outputs: dict = None
+    host: dict = field(default_factory=dict)
+
+
+class ServingEngine:
+    def __init__(self, ctx, slots=4, reserve_bytes=2 << 30, free_bytes_fn=None):
+        self.ctx = ctx
+        self.slots = [_Slot(stream=torch.cuda.Stream(device=ctx.device)) for _ in range(slots)]
+        self.queue = deque()
+        self.finished = []
+        self.plans = {}
+        self.reserve = int(reserve_bytes)
+        self.free_bytes_fn = free_bytes_fn or self._free_bytes
+        self.t0 = time.perf_counter()
+        self.deferred = 0
+
+    # ---------------------------------------------------------------- memory / plans
+    def _free_bytes(self):
+        free, _ = torch.cuda.mem_get_info(self.ctx.device)
+        stats = torch.cuda.memory_stats(self.ctx.device)
+        cached = stats.get("reserved_bytes.all.current", 0) - stats.get("allocated_bytes.all.current", 0)
+        return free + max(0, cached)
+
+    def plan(self, m):
+        p = self.plans.get(m)
+        if p is None:
+            p = self.plans[m] = CataloguePlan(self.ctx, m)
+        return p
+
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    # ---------------------------------------------------------------- API
+    def submit(self, req):
+        if req.arrival is None or req.arrival == 0.0:
+            req.arrival = self.now()
+        self.queue.append(req)
+
+    def in_flight(self):
+        return sum(1 for s in self.slots if s.req is not None)
+
+    def poll(self):
+        """Retire finished requests, then admit queued ones (FIFO) while slots and memory
+        allow.  Returns the requests finished by this call."""
+        done = []
+        for s in self.slots:
+            if s.req is not None and s.event.query():
+                r = s.req
+                r.done = self.now()
+                r.result = {f: (s.host[f][0].numpy().copy(), s.host[f][1].numpy().copy())
+                            for f in s.host}
+                s.req, s.outputs, s.host, s.event = None, None, {}, None
+                done.append(r)
+        self.finished.extend(done)
+        while self.queue:
+            r = self.queue[0]
+            if r.arrival > self.now():
+                break
+            slot = next((s for s in self.slots if s.req is None), None)
+            if slot is None:
+                break
+            plan = self.plan(r.measurement)
+            need = plan.bytes_per_catalogue(r.npart) + self.reserve
+            if need > self.free_bytes_fn():
+                if self.in_flight() == 0:
+                    raise MemoryError(f"request {r.rid} needs {need} bytes, more than the device has")
+                self.deferred += 1
+                break
+            self.queue.popleft()
+            self._launch(slot, r, plan)
+        return done
+
+    def _launch(self, slot, r, plan):
+        r.admitted = self.now()
+        slot.req = r
+        # the request's inputs are produced on the default stream
+        slot.stream.wait_stream(torch.cuda.current_stream(self.ctx.device))
+        slot.outputs = plan.run(r.pos, r.vel, r.npart, stream=slot.stream)
+        with torch.cuda.stream(slot.stream):
+            for f, (pout, bout) in slot.outputs.items():
+                hp = torch.empty(pout.shape, dtype=pout.dtype, pin_memory=True)
+                hb = torch.empty(bout.shape, dtype=bout.dtype, pin_memory=True)
+                hp.copy_(pout, non_blocking=True)
+                hb.copy_(bout, non_blocking=True)
+                slot.host[f] = (hp, hb)
+            slot.event = torch.cuda.Event()
+            slot.event.record(slot.stream)
+
+    def run(self, requests=None, poll_sleep=50e-6):
+        """Submit `requests` (their .arrival are offsets from now) and serve until all are
+        finished.  Returns the finished requests in completion order."""
+        base = self.now()
+        n0 = len(self.finished)
+        for r in requests or []:
+            r.arrival = base + (r.arrival or 0.0)
+            self.queue.append(r)
+        while self.queue or self.in_flight():
+            self.poll()
+            if self.queue or self.in_flight():
+                time.sleep(poll_sleep)
+        return self.finished[n0:]
+
+
+def latency_summary(reqs):
+    lat = np.array([r.latency for r in reqs if r.latency is not None]) * 1e3
+    if lat.size == 0:
+        return {}
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "n": int(lat.size)}
