@@ -85,6 +85,18 @@ size_t pbi4_density_workspace(pbi4_ctx *ctx, const pbi4_density_desc *d);
 int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float *grids,
                          void *delta, void *ws, size_t ws_bytes, void *stream);
 
+/* Full-array forms of the reference routines for callers that hold complete N^3 arrays (the
+ * f2py-named Python functions).  Arrays are complex float2 [z][y][x] (Fortran (x,y,z) order).
+ * pbi4_fcomb: powerI4.fcomb (powerI4.f90:251-378) on fdcl = FFT(grid1 + i grid2), all N^3
+ * outputs written with the reference's ordered-store semantics (dcl /= npart is the caller's).
+ * pbi4_compute_qk: powerI4.compute_q2k (norder 2, idx[0..1]) and compute_q4k (norder 4,
+ * idx[0..3]) (powerI4.f90:409-508); idx are 1-based axes as in the reference.
+ */
+int pbi4_fcomb(pbi4_ctx *ctx, int grid, int nintp, int interlacing, float box, const void *fdcl,
+               void *dcl, void *stream);
+int pbi4_compute_qk(pbi4_ctx *ctx, int grid, int norder, const int *idx, const void *deltak,
+                    void *out, void *stream);
+
 /* ------------------------------------------------------------------ power spectrum
  * Replaces powerI4.measure_pk (powerI4.f90:510-594, multipoles = 0) and
  * powerI4.measure_pk_multipoles (powerI4.f90:596-693, multipoles = 1) together with
@@ -154,6 +166,16 @@ int pbi4_bk_maps_grid(pbi4_ctx *ctx, const pbi4_bk_desc *d);
 int pbi4_measure_bk(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *counts, const void *delta,
                     const void *delta2, const void *delta4, double *bout, void *ws,
                     size_t ws_bytes, void *stream);
+
+/* ------------------------------------------------------------------ fused P_l + B_l
+ * pbi4_measure_pk followed by pbi4_measure_bk on the same delta, with one binning pass that
+ * produces the P_l bins and the shell powers together (the serving path).  Same outputs as
+ * the two calls.  The descriptors must agree on grid and multipoles; L_l(mu) is applied.
+ */
+size_t pbi4_pk_bk_workspace(pbi4_ctx *ctx, const pbi4_pk_desc *pd, const pbi4_bk_desc *d);
+int pbi4_measure_pk_bk(pbi4_ctx *ctx, const pbi4_pk_desc *pd, const pbi4_bk_desc *d,
+                       const float *counts, const void *delta, double *pout, double *bout,
+                       void *ws, size_t ws_bytes, void *stream);
 
 /* ------------------------------------------------------------------ triangle sums
  * The triple-sum loops of bispectrumI4.f90:164-193 (measure_bk), :476-511

@@ -1,7 +1,7 @@
 ! C-callable entry points into the reference's own powerI4.f90, compiled unchanged from
 ! /root/reference/pbi4/F90_modules/powerI4.f90 by oracle/Makefile into oracle/_ref/.
 ! TEST INFRASTRUCTURE ONLY: this shim lets the CPU tests check the oracle restatement against
-! the reference routines (assign_single_grid, assign_double_grid, fcomb, compute_q2k,
+! the reference routines (assign_single_grid, assign_double_grid, fcomb, reduce_array, compute_q2k,
 ! compute_q4k, measure_pk, measure_pk_multipoles).  The product library never loads it.
 ! Arrays are passed as plain pointers with explicit sizes; the reference routines take
 ! assumed-shape dummies, so each call goes through an explicit interface.
@@ -27,6 +27,11 @@ module ref_bind
       logical, intent(in) :: interlacing
       complex(kind=8), intent(in) :: fdcl(:,:,:)
       complex(kind=8), intent(out) :: dcl(grid, grid, grid)
+    end subroutine
+    subroutine reduce_array(L, din, grid, dout)
+      integer, intent(in) :: L, grid
+      complex(kind=8), intent(in) :: din(:,:,:)
+      complex(kind=8), intent(out) :: dout(grid, grid, grid)
     end subroutine
     subroutine compute_q2k(grid, box, deltak, i, j, q2k)
       integer, intent(in) :: grid, i, j
@@ -80,6 +85,14 @@ contains
     logical :: il
     il = interlacing /= 0
     call fcomb(grid, nintp, il, box, fdcl, dcl)
+  end subroutine
+
+  subroutine ref_reduce_array(L, din, grid, dout) bind(C)
+    integer(c_int), value :: L, grid
+    complex(c_double_complex), intent(in) :: din(L, L, L)
+    complex(c_double_complex), intent(out) :: dout(grid, grid, grid)
+    dout = (0.d0, 0.d0)
+    call reduce_array(L, din, grid, dout)
   end subroutine
 
   subroutine ref_compute_q2k(grid, box, deltak, i, j, q2k) bind(C)

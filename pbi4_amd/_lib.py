@@ -61,6 +61,8 @@ SIGNATURES = {
     "pbi4_density_workspace": (c_size_t, [c_void_p, P(DensityDesc)]),
     "pbi4_compute_density": (c_int, [c_void_p, P(DensityDesc), c_void_p, c_void_p, c_void_p,
                                      c_size_t, c_void_p]),
+    "pbi4_fcomb": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_void_p, c_void_p]),
+    "pbi4_compute_qk": (c_int, [c_void_p, c_int, c_int, P(c_int), c_void_p, c_void_p, c_void_p]),
     "pbi4_pk_workspace": (c_size_t, [c_void_p, P(PkDesc)]),
     "pbi4_measure_pk": (c_int, [c_void_p, P(PkDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_size_t, c_void_p]),
@@ -70,6 +72,9 @@ SIGNATURES = {
     "pbi4_bk_maps_grid": (c_int, [c_void_p, P(BkDesc)]),
     "pbi4_measure_bk": (c_int, [c_void_p, P(BkDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pbi4_pk_bk_workspace": (c_size_t, [c_void_p, P(PkDesc), P(BkDesc)]),
+    "pbi4_measure_pk_bk": (c_int, [c_void_p, P(PkDesc), P(BkDesc), c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_size_t, c_void_p]),
     "pbi4_triangle_sums_workspace": (c_size_t, [c_int, c_int, c_int64, c_int64, c_int]),
     "pbi4_triangle_sums": (c_int, [c_void_p, c_int, c_int, c_int64, c_int64, c_int, c_void_p,
                                    c_void_p, c_void_p, c_size_t, c_void_p]),

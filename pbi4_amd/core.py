@@ -102,6 +102,26 @@ class Context:
                                                _ptr(ws), ws.numel(), _stream(stream)))
         return out
 
+    # ------------------------------------------------------------- full-array forms
+    def fcomb_full(self, fdcl, grid, nintp, interlacing, box, out=None, stream=None):
+        """powerI4.fcomb on a full (N, N, N) complex64 [z][y][x] array."""
+        _check_dev(fdcl, "fdcl", torch.complex64)
+        if out is None:
+            out = torch.empty_like(fdcl)
+        self._check(lib().pbi4_fcomb(self.h, int(grid), int(nintp), int(bool(interlacing)),
+                                     float(box), _ptr(fdcl), _ptr(out), _stream(stream)))
+        return out
+
+    def compute_qk(self, deltak, grid, idx, out=None, stream=None):
+        """compute_q2k (len(idx) == 2) / compute_q4k (len(idx) == 4), 1-based axes."""
+        _check_dev(deltak, "deltak", torch.complex64)
+        if out is None:
+            out = torch.empty_like(deltak)
+        arr = (ctypes.c_int * 4)(*(list(idx) + [1] * (4 - len(idx))))
+        self._check(lib().pbi4_compute_qk(self.h, int(grid), len(idx), arr, _ptr(deltak), _ptr(out),
+                                          _stream(stream)))
+        return out
+
     # ------------------------------------------------------------- P(k)
     def pk(self, delta, grid, box, kbin, kcenter, nbins, multipoles=False, los=(0.0, 0.0, 1.0),
            delta_b=None, delta2=None, delta4=None, out=None, stream=None):
@@ -166,6 +186,33 @@ class Context:
                                           _ptr(delta2), _ptr(delta4), _ptr(out), _ptr(ws),
                                           ws.numel(), _stream(stream)))
         return out
+
+    # ------------------------------------------------------------- fused P_l + B_l
+    def pk_bk_workspace(self, pk_desc, bk_desc):
+        n = lib().pbi4_pk_bk_workspace(self.h, ctypes.byref(pk_desc), ctypes.byref(bk_desc))
+        if n == 0:
+            raise Pbi4Error("pbi4_pk_bk_workspace: " + lib().pbi4_last_error(self.h).decode())
+        return n
+
+    def pk_desc(self, grid, box, kbin, kcenter, nbins, multipoles=False, los=(0.0, 0.0, 1.0)):
+        return PkDesc(int(grid), float(box), float(kbin), float(kcenter), int(nbins),
+                      int(bool(multipoles)), (ctypes.c_double * 3)(*[float(x) for x in los]))
+
+    def pk_bk(self, delta, counts, pk_desc, bk_desc, pout=None, bout=None, ws=None, stream=None):
+        """measure_pk + measure_bk on one delta with a single binning pass (serving path)."""
+        _check_dev(delta, "delta", torch.complex64)
+        _check_dev(counts, "counts", torch.float32)
+        rows = 11 if bk_desc.multipoles else 9
+        if pout is None:
+            pout = torch.empty((pk_desc.nbins, 5), dtype=torch.float64, device=self.device)
+        if bout is None:
+            bout = torch.empty((bk_desc.itot, rows), dtype=torch.float64, device=self.device)
+        if ws is None:
+            ws = self._ws(self.pk_bk_workspace(pk_desc, bk_desc))
+        self._check(lib().pbi4_measure_pk_bk(self.h, ctypes.byref(pk_desc), ctypes.byref(bk_desc),
+                                             _ptr(counts), _ptr(delta), _ptr(pout), _ptr(bout),
+                                             _ptr(ws), ws.numel(), _stream(stream)))
+        return pout, bout
 
     # ------------------------------------------------------------- triangle sums
     def triangle_sums(self, maps, nb, nell, K, precision, stream=None):

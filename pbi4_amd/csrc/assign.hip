@@ -65,7 +65,7 @@ __device__ __forceinline__ float norm_pos(const PosXform &x, const float *pos, c
     float p = pos[(int64_t)d * npart + i];
     if (x.pos_mode == PBI4_POS_NORMALISED) return p;
     float s = p;
-    if (x.has_vel) s = fmaf(x.rsd[d], vel[(int64_t)d * npart + i], s);
+    if (x.has_vel && x.rsd[d] != 0.0f) s = fmaf(x.rsd[d], vel[(int64_t)d * npart + i], s);
     float t = fmodf(s + x.box, x.box);
     if (t < 0.0f) t += x.box;
     float q = t / x.box;
@@ -93,6 +93,32 @@ __device__ __forceinline__ int tile_of(const PosXform &x, int T, int NT, const f
 
 // ------------------------------------------------------------------ counting sort
 
+// Runs of equal keys among consecutive lanes of a wave (catalogues stored in spatial order put
+// neighbouring particles in the same tile): rank = position in the run, len = run length at the
+// run's first lane.  One LDS atomic per run instead of one per lane.
+__device__ __forceinline__ void run_of(int key, int &rank, int &len) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const bool head = lane == 0 || prev != key;
+    // index of the run head at or below this lane: inclusive max-scan of head positions
+    int h = head ? lane : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(h, off);
+        if (lane >= off) h = max(h, o);
+    }
+    rank = lane - h;
+    // run length: next head above (or 64) minus this head, valid at the head
+    int nh = head ? lane : 64;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_down(nh, off);
+        if (lane + off < 64) nh = min(nh, o);
+    }
+    const int next = __shfl_down(nh, 1);
+    len = (lane + 1 < 64 ? next : 64) - lane;
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, int NT,
                                                             int64_t ntiles, const float *pos,
                                                             const float *vel, int64_t npart,
@@ -104,8 +130,13 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, i
     if (use_lds) {
         for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
         __syncthreads();
-        for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
-            atomicAdd(&lhist[tile_of(x, T, NT, pos, vel, i, npart, nullptr)], 1u);
+        for (int64_t i0 = beg; i0 < end; i0 += blockDim.x) {
+            const int64_t i = i0 + threadIdx.x;
+            const int t = i < end ? tile_of(x, T, NT, pos, vel, i, npart, nullptr) : -1;
+            int rank, len;
+            run_of(t, rank, len);
+            if (t >= 0 && rank == 0) atomicAdd(&lhist[t], (uint32_t)len);
+        }
         __syncthreads();
         for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x)
             if (lhist[t]) atomicAdd(&counts[t], lhist[t]);
@@ -173,11 +204,12 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
 #pragma unroll
         for (int k = 0; k < kScatterPer; ++k) {
             int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
-            tl[k] = -1;
-            if (i < end) {
-                tl[k] = tile_of(x, T, NT, pos, vel, i, npart, pk[k]);
-                rank[k] = atomicAdd(&lhist[tl[k]], 1u);
-            }
+            tl[k] = i < end ? tile_of(x, T, NT, pos, vel, i, npart, pk[k]) : -1;
+            int rr, len;
+            run_of(tl[k], rr, len);
+            uint32_t base = 0;
+            if (tl[k] >= 0 && rr == 0) base = atomicAdd(&lhist[tl[k]], (uint32_t)len);
+            rank[k] = __shfl(base, (threadIdx.x & 63) - rr) + (uint32_t)rr;
         }
         __syncthreads();
         for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
@@ -260,6 +292,22 @@ __device__ __forceinline__ void stencil(int grid_sel, int single, int c, float f
     }
 }
 
+// LDS accumulation is 32-bit fixed point.  A round deposits up to kPer(ORDER) particles per
+// thread; the scale 2^-s is chosen so that no LDS cell can overflow within a round: the largest
+// single contribution is (2/3)^3 for PCS, (3/4)^3 for TSC, 1 for CIC / NGP and 1.5^3 for the
+// single-grid CIC of powerI4.f90:114-124 (h = vr - nint(vr) gives weights in [-0.5, 1.5]).
+// After each round the LDS tile is written to (first round) or added to (later rounds, fixed
+// order) the tile's region; integer sums within a round are order independent, so the grids
+// are bitwise reproducible.
+template <int ORDER>
+__device__ __forceinline__ int deposit_per_thread(int single) {
+    return (ORDER >= 3) ? 4 : (ORDER == 2 && single) ? 1 : 2;
+}
+template <int ORDER>
+__device__ __forceinline__ float deposit_scale(int single) {
+    return (ORDER >= 3) ? 1048576.0f : (ORDER == 2 && single) ? 262144.0f : 524288.0f;
+}
+
 template <int ORDER>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int ngrids,
                                                              const uint32_t *offsets,
@@ -267,85 +315,116 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int n
                                                              const uint32_t *sy,
                                                              const uint32_t *sz,
                                                              float *regions) {
-    __shared__ unsigned long long acc[2 * kMaxR * kMaxR * kMaxR];
-    const int R3 = R * R * R;
-    const int64_t tile = blockIdx.x;
-    for (int i = threadIdx.x; i < ngrids * R3; i += blockDim.x) acc[i] = 0ull;
-    __syncthreads();
-    const uint32_t b = offsets[tile], e = offsets[tile + 1];
+    __shared__ int acc[2 * kMaxR * kMaxR * kMaxR];
+    const int R3 = R * R * R, ncell = ngrids * R3;
     const int single = ngrids == 1;
-    for (uint32_t p = b + threadIdx.x; p < e; p += blockDim.x) {
-        uint32_t q[3] = {sx[p], sy[p], sz[p]};
-        int c[3];
-        float f[3];
-        for (int d = 0; d < 3; ++d) {
-            c[d] = (int)(q[d] >> 28);
-            f[d] = (float)(q[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
-        }
-        for (int g = 0; g < ngrids; ++g) {
-            int st[3];
-            float w[3][4];
-            for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
-            unsigned long long *a = acc + g * R3;
+    const int per = deposit_per_thread<ORDER>(single);
+    const float fix = deposit_scale<ORDER>(single);
+    const float inv = 1.0f / fix;
+    const int64_t tile = blockIdx.x;
+    float *out = regions + tile * (int64_t)ncell;
+    const uint32_t b = offsets[tile], e = offsets[tile + 1];
+    const uint32_t round = (uint32_t)blockDim.x * per;
+    bool first = true;
+    uint32_t r0 = b;
+    do {
+        for (int i = threadIdx.x; i < ncell; i += blockDim.x) acc[i] = 0;
+        __syncthreads();
+        // lane-major swizzle: the 64 lanes of a wave take particles 16 apart in tile order, so
+        // neighbours in the sort (often neighbours in space) land in different waves and their
+        // stencils rarely hit the same LDS address within one atomic instruction
+        const uint32_t nw = blockDim.x >> 6;
+        const uint32_t sw = (threadIdx.x & 63) * nw + (threadIdx.x >> 6);
+        for (int j = 0; j < per; ++j) {
+            const uint32_t p = r0 + sw + (uint32_t)j * blockDim.x;
+            if (p >= e) continue;
+            const uint32_t q[3] = {sx[p], sy[p], sz[p]};
+            int c[3];
+            float f[3];
+            for (int d = 0; d < 3; ++d) {
+                c[d] = (int)(q[d] >> 28);
+                f[d] = (float)(q[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
+            }
+            for (int g = 0; g < ngrids; ++g) {
+                int st[3];
+                float w[3][4];
+                for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
+                int *a = acc + g * R3;
+#pragma unroll 1
+                for (int iz = 0; iz < ORDER; ++iz) {
 #pragma unroll
-            for (int ix = 0; ix < ORDER; ++ix) {
+                    for (int iy = 0; iy < ORDER; ++iy) {
+                        const float wyz = w[1][iy] * w[2][iz];
+                        const int row = ((st[2] + kHalo + iz) * R + st[1] + kHalo + iy) * R + st[0] + kHalo;
 #pragma unroll
-                for (int iy = 0; iy < ORDER; ++iy) {
-                    const float wxy = w[0][ix] * w[1][iy];
-                    const int row = (st[1] + kHalo + iy) * R + st[0] + kHalo + ix;
-#pragma unroll
-                    for (int iz = 0; iz < ORDER; ++iz) {
-                        const float v = wxy * w[2][iz];
-                        const long long fx = __float2ll_rn(v * kFix);
-                        atomicAdd(&a[(st[2] + kHalo + iz) * R * R + row],
-                                  (unsigned long long)fx);
+                        for (int ix = 0; ix < ORDER; ++ix)
+                            atomicAdd(&a[row + ix], __float2int_rn((w[0][ix] * wyz) * fix));
                     }
                 }
             }
         }
-    }
-    __syncthreads();
-    float *out = regions + tile * (int64_t)ngrids * R3;
-    for (int i = threadIdx.x; i < ngrids * R3; i += blockDim.x)
-        out[i] = __ll2float_rn((long long)acc[i]) * kInvFix;
+        __syncthreads();
+        if (first) {
+            for (int i = threadIdx.x; i < ncell; i += blockDim.x) out[i] = (float)acc[i] * inv;
+        } else {
+            for (int i = threadIdx.x; i < ncell; i += blockDim.x) out[i] += (float)acc[i] * inv;
+        }
+        first = false;
+        r0 += round;
+        __syncthreads();
+    } while (r0 < e);
 }
 
-// grid[g][z][y][x] = sum of the region entries covering (x,y,z), in a fixed order.
-__global__ void k_gather(int N, int T, int NT, int R, int ngrids, const float *regions,
-                         float *grids) {
-    const int64_t V = (int64_t)N * N * N;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= V) return;
-    const int x = (int)(idx % N), y = (int)((idx / N) % N), z = (int)(idx / ((int64_t)N * N));
-    const int pos[3] = {x, y, z};
-    int nt[3][3], ni[3][3], cnt[3];
-    for (int d = 0; d < 3; ++d) {
-        const int t = pos[d] / T, c = pos[d] - t * T;
-        int k = 0;
-        nt[d][k] = t;
-        ni[d][k++] = c + kHalo;
-        if (c < kHalo) {
-            nt[d][k] = (t + NT - 1) % NT;
-            ni[d][k++] = c + T + kHalo;
-        }
-        if (c >= T - kHalo) {
-            nt[d][k] = (t + 1) % NT;
-            ni[d][k++] = c - T + kHalo;
-        }
-        cnt[d] = k;
+// grid[g][z][y][x] = sum of the region entries covering (x,y,z) in a fixed order (home tile,
+// then the previous / next tile along each axis).  One work-group per (tile, grid): thread
+// (x, y) of the T x T face walks z.
+__global__ __launch_bounds__(256) void k_gather(int N, int T, int NT, int R, int ngrids,
+                                                const float *regions, float *grids) {
+    const int tile = blockIdx.x, g = blockIdx.y;
+    const int tx = tile % NT, ty = (tile / NT) % NT, tz = tile / (NT * NT);
+    const int cx = threadIdx.x % 16, cy = threadIdx.x / 16;
+    if (cx >= T || cy >= T) return;
+    const int R3 = R * R * R;
+    int ntx[2], nix[2], nx = 1, nty[2], niy[2], ny = 1;
+    ntx[0] = tx;
+    nix[0] = cx + kHalo;
+    if (cx < kHalo) {
+        ntx[nx] = (tx + NT - 1) % NT;
+        nix[nx++] = cx + T + kHalo;
+    } else if (cx >= T - kHalo) {
+        ntx[nx] = (tx + 1) % NT;
+        nix[nx++] = cx - T + kHalo;
     }
-    const int64_t R3 = (int64_t)R * R * R;
-    for (int g = 0; g < ngrids; ++g) {
+    nty[0] = ty;
+    niy[0] = cy + kHalo;
+    if (cy < kHalo) {
+        nty[ny] = (ty + NT - 1) % NT;
+        niy[ny++] = cy + T + kHalo;
+    } else if (cy >= T - kHalo) {
+        nty[ny] = (ty + 1) % NT;
+        niy[ny++] = cy - T + kHalo;
+    }
+    const int64_t V = (int64_t)N * N * N;
+    float *out = grids + (int64_t)g * V + ((int64_t)(ty * T + cy) * N + tx * T + cx);
+    for (int cz = 0; cz < T; ++cz) {
+        int ntz[2], niz[2], nz = 1;
+        ntz[0] = tz;
+        niz[0] = cz + kHalo;
+        if (cz < kHalo) {
+            ntz[nz] = (tz + NT - 1) % NT;
+            niz[nz++] = cz + T + kHalo;
+        } else if (cz >= T - kHalo) {
+            ntz[nz] = (tz + 1) % NT;
+            niz[nz++] = cz - T + kHalo;
+        }
         float s = 0.0f;
-        for (int a = 0; a < cnt[2]; ++a)
-            for (int b = 0; b < cnt[1]; ++b)
-                for (int c = 0; c < cnt[0]; ++c) {
-                    const int64_t tile = ((int64_t)nt[2][a] * NT + nt[1][b]) * NT + nt[0][c];
-                    const int64_t o = (tile * ngrids + g) * R3 +
-                                      ((int64_t)ni[2][a] * R + ni[1][b]) * R + ni[0][c];
-                    s += regions[o];
+        for (int a = 0; a < nz; ++a)
+            for (int bb = 0; bb < ny; ++bb)
+                for (int c = 0; c < nx; ++c) {
+                    const int64_t t = ((int64_t)ntz[a] * NT + nty[bb]) * NT + ntx[c];
+                    s += regions[(t * ngrids + g) * R3 + (niz[a] * R + niy[bb]) * R + nix[c]];
                 }
-        grids[(int64_t)g * V + idx] = s;
+        out[(int64_t)(tz * T + cz) * N * N] = s;
     }
 }
 
@@ -454,9 +533,11 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
             default: launch(k_deposit<4>); break;
         }
     }
-    const int64_t V = (int64_t)p.N * p.N * p.N;
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)ceil_div(V, 256)), dim3(256), 0, s, p.N, p.T,
-                       p.NT, p.R, p.ngrids, regions, grids);
+    {
+        TimedScope ts(timer, "pbi4_gather", s);
+        hipLaunchKernelGGL(k_gather, dim3((unsigned)p.ntiles, p.ngrids), dim3(256), 0, s, p.N, p.T,
+                           p.NT, p.R, p.ngrids, regions, grids);
+    }
     PBI4_HIP(hipGetLastError());
 }
 

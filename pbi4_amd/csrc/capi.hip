@@ -382,7 +382,7 @@ void check_bk(const pbi4_bk_desc &d) {
                PBI4_EINVAL, "maps_grid must be 0 or an even size <= grid");
 }
 
-BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
+BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a, int nbins_pk = 0) {
     ShellInfo &sh = get_shells(ctx, d.grid, d.nbins, d.kbin, d.kcenter);
     BkLayout L{};
     L.Np = d.maps_grid ? d.maps_grid : choose_maps_grid(d.grid, sh.R);
@@ -394,7 +394,7 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     L.K = (int64_t)L.Np * L.Np * L.Np;
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, L.nell, L.K, L.Kstride, d.precision);
-    BinPlan b = make_bin_plan(d.grid, 0, d.nbins, 0, d.los, 1.0f);
+    BinPlan b = make_bin_plan(d.grid, nbins_pk, d.nbins, 0, d.los, 1.0f);
     FftPlan &p = get_plan(ctx, 1, L.Np, L.nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(d.nbins);
@@ -407,6 +407,47 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     return L;
 }
 
+// Shell filter -> c2r maps -> triangle GEMM -> FP64 finalize, once the shell sums are in L.
+void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo &sh, int32_t *tri,
+                    const float *counts, const void *delta, const void *delta2, const void *delta4,
+                    double *bout, hipStream_t s) {
+    FilterPlan f{};
+    f.N = d->grid;
+    f.Np = L.Np;
+    f.R = sh.R;
+    f.nshell = d->nbins;
+    f.nell = L.nell;
+    for (int i = 0; i < 3; ++i) f.los[i] = (float)d->los[i];
+    f.shell_table = sh.table_d;
+    f.max_d2 = sh.max_d2;
+    f.f2 = static_cast<const float2 *>(delta2);
+    f.f4 = static_cast<const float2 *>(delta4);
+    PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
+               "delta2/delta4 are only used with multipoles");
+    {
+        TimedScope ts(&ctx->timer, "pbi4_filter", s);
+        launch_filter(f, static_cast<const float2 *>(delta), L.shell_sum, sh.nmodes_d, L.spectra,
+                      L.scale_exp, s);
+    }
+    FftPlan &p = get_plan(ctx, 1, L.Np, L.nmaps, L.Kstride);
+    {
+        TimedScope ts(&ctx->timer, "pbi4_c2r", s);
+        fft_exec(p, L.spectra, L.maps, L.fftwork, s);
+    }
+    launch_gemm(L.g, L.maps, L.gemm_partials, s, &ctx->timer);
+    launch_gemm_reduce(L.g, L.gemm_partials, L.C, s);
+    // kf3 = kf**3 in REAL(4) (bispectrumI4.f90:46); powSN = 1/dble(npart)/kf3 (:47)
+    volatile float kfv = d->kf;
+    volatile float k2 = kfv * kfv;
+    volatile float k3 = k2 * kfv;
+    const double kf3 = (double)k3;
+    const double powSN = 1.0 / (double)d->npart / kf3;
+    const double gridD = (double)d->grid * d->grid * d->grid;
+    const double ratio = (double)d->grid / (double)L.Np;
+    launch_bk_finalize(d->multipoles ? 11 : 9, d->itot, tri, L.C, L.g.npair, L.g.ncol, d->nbins,
+                       L.scale_exp, L.shell_sum, sh.nmodes_d, counts, kf3, powSN, gridD,
+                       ratio * ratio * ratio, (double)d->kcenter, (double)d->kbin, bout, s);
+}
 }  // namespace
 
 // ==================================================================== C ABI
@@ -502,6 +543,33 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
         TimedScope ts(&ctx->timer, "pbi4_combine", s);
         launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order), (float)cf, w.G,
                        w.G + half, static_cast<float2 *>(delta), s);
+    });
+}
+
+int pbi4_fcomb(pbi4_ctx *ctx, int grid, int nintp, int interlacing, float box, const void *fdcl,
+               void *dcl, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(fdcl && dcl && fdcl != dcl, PBI4_EINVAL, "fdcl and dcl must be distinct buffers");
+        PBI4_CHECK(grid >= 2 && grid % 2 == 0, PBI4_EINVAL, "grid must be even");
+        PBI4_CHECK(nintp >= 1 && nintp <= 4 && box > 0.0f, PBI4_EINVAL, "bad nintp / box");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        const double kF3 = kf3_fcomb(box);
+        const double cf = interlacing ? 1.0 / (4.0 * kF3) : 1.0 / kF3;
+        launch_fcomb_full(grid, interlacing, get_window(ctx, grid, nintp), (float)cf,
+                          static_cast<const float2 *>(fdcl), static_cast<float2 *>(dcl),
+                          (hipStream_t)stream);
+    });
+}
+
+int pbi4_compute_qk(pbi4_ctx *ctx, int grid, int norder, const int *idx, const void *deltak,
+                    void *out, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(idx && deltak && out, PBI4_EINVAL, "NULL argument");
+        PBI4_CHECK(norder == 2 || norder == 4, PBI4_EINVAL, "norder must be 2 or 4");
+        for (int i = 0; i < norder; ++i)
+            PBI4_CHECK(idx[i] >= 1 && idx[i] <= 3, PBI4_EINVAL, "indices are 1..3");
+        launch_qk(grid, norder, idx, static_cast<const float2 *>(deltak), static_cast<float2 *>(out),
+                  (hipStream_t)stream);
     });
 }
 
@@ -662,42 +730,50 @@ int pbi4_measure_bk(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *counts, c
         b.max_d2 = sh.max_d2;
         launch_bin(b, static_cast<const float2 *>(delta), L.bin_partials, s);
         launch_bin_finalize(b, L.bin_partials, 1.0, nullptr, L.shell_sum, s);
-        FilterPlan f{};
-        f.N = d->grid;
-        f.Np = L.Np;
-        f.R = sh.R;
-        f.nshell = d->nbins;
-        f.nell = L.nell;
-        for (int i = 0; i < 3; ++i) f.los[i] = (float)d->los[i];
-        f.shell_table = sh.table_d;
-        f.max_d2 = sh.max_d2;
-        f.f2 = static_cast<const float2 *>(delta2);
-        f.f4 = static_cast<const float2 *>(delta4);
-        PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
-                   "delta2/delta4 are only used with multipoles");
+        bk_from_shells(ctx, d, L, sh, tri, counts, delta, delta2, delta4, bout, s);
+    });
+}
+
+size_t pbi4_pk_bk_workspace(pbi4_ctx *ctx, const pbi4_pk_desc *pd, const pbi4_bk_desc *d) {
+    size_t r = 0;
+    guarded(ctx, [&] {
+        check_bk(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        Arena a(nullptr, 0);
+        bk_layout(ctx, *d, a, pd->nbins);
+        r = a.used + 256;
+    });
+    return r;
+}
+
+int pbi4_measure_pk_bk(pbi4_ctx *ctx, const pbi4_pk_desc *pd, const pbi4_bk_desc *d,
+                       const float *counts, const void *delta, double *pout, double *bout,
+                       void *ws, size_t ws_bytes, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(pd && d && counts && delta && pout && bout && ws, PBI4_EINVAL, "NULL argument");
+        check_bk(*d);
+        PBI4_CHECK(pd->grid == d->grid && pd->multipoles == d->multipoles, PBI4_EINVAL,
+                   "P(k) and bispectrum descriptors disagree on grid / multipoles");
+        PBI4_CHECK(pd->box > 0.0f, PBI4_EINVAL, "box must be > 0");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipStream_t s = (hipStream_t)stream;
+        ShellInfo &sh = get_shells(ctx, d->grid, d->nbins, d->kbin, d->kcenter);
+        PkTable &t = get_pk_table(ctx, pd->grid, pd->box, pd->kbin, pd->kcenter, pd->nbins);
+        int32_t *tri = get_tri_table(ctx, d->nbins, d->kbin, d->kcenter, d->iopen, 0, d->itot);
+        Arena a(ws, ws_bytes);
+        BkLayout L = bk_layout(ctx, *d, a, pd->nbins);
+        // one pass over delta: P_l bins and shell powers together
+        BinPlan b = make_bin_plan(d->grid, pd->nbins, d->nbins, pd->multipoles, pd->los, pd->box);
+        b.pk_table = t.d;
+        b.shell_table = sh.table_d;
+        b.max_d2 = sh.max_d2;
         {
-            TimedScope ts(&ctx->timer, "pbi4_filter", s);
-            launch_filter(f, static_cast<const float2 *>(delta), L.shell_sum, sh.nmodes_d, L.spectra,
-                          L.scale_exp, s);
+            TimedScope ts(&ctx->timer, "pbi4_bin", s);
+            launch_bin(b, static_cast<const float2 *>(delta), L.bin_partials, s);
         }
-        FftPlan &p = get_plan(ctx, 1, L.Np, L.nmaps, L.Kstride);
-        {
-            TimedScope ts(&ctx->timer, "pbi4_c2r", s);
-            fft_exec(p, L.spectra, L.maps, L.fftwork, s);
-        }
-        launch_gemm(L.g, L.maps, L.gemm_partials, s, &ctx->timer);
-        launch_gemm_reduce(L.g, L.gemm_partials, L.C, s);
-        // kf3 = kf**3 in REAL(4) (bispectrumI4.f90:46); powSN = 1/dble(npart)/kf3 (:47)
-        volatile float kfv = d->kf;
-        volatile float k2 = kfv * kfv;
-        volatile float k3 = k2 * kfv;
-        const double kf3 = (double)k3;
-        const double powSN = 1.0 / (double)d->npart / kf3;
-        const double gridD = (double)d->grid * d->grid * d->grid;
-        const double ratio = (double)d->grid / (double)L.Np;
-        launch_bk_finalize(d->multipoles ? 11 : 9, d->itot, tri, L.C, L.g.npair, L.g.ncol, d->nbins,
-                           L.scale_exp, L.shell_sum, sh.nmodes_d, counts, kf3, powSN, gridD,
-                           ratio * ratio * ratio, (double)d->kcenter, (double)d->kbin, bout, s);
+        const double kF = (double)(kTwoPiF32 / pd->box);
+        launch_bin_finalize(b, L.bin_partials, kF * kF * kF, pout, L.shell_sum, s);
+        bk_from_shells(ctx, d, L, sh, tri, counts, delta, nullptr, nullptr, bout, s);
     });
 }
 

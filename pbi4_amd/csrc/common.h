@@ -74,6 +74,10 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
 void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
                     const float2 *G2, float2 *delta, hipStream_t s);
 
+void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
+                       float2 *out, hipStream_t s);
+void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, hipStream_t s);
+
 struct BinPlan {
     int N;                    // grid
     int nbins_pk;             // P(k) bins (0 = none)

@@ -214,14 +214,35 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
     }
 }
 
-__global__ void k_bin_finalize(BinPlan b, const float *partials, double kF3, double *pk_out,
-                               double *shell_sum) {
+// FP64 combine of the per-work-group partials: 32 bins per block, the work-group range split
+// over 8 threads in fixed sub-ranges and summed in a fixed order; then pout / shell sums.
+__global__ __launch_bounds__(256) void k_bin_finalize(BinPlan b, const float *partials,
+                                                      double kF3, double *pk_out,
+                                                      double *shell_sum) {
+    __shared__ double red[8][32][5];
     const int nq = 5 * b.nbins_pk + b.nshell;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lo = threadIdx.x & 31, part = threadIdx.x >> 5;
+    const int i = blockIdx.x * 32 + lo;
+    const int per = (b.nwg + 7) / 8, g0 = part * per, g1 = min(b.nwg, g0 + per);
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < b.nbins_pk) {
-        double s[5] = {0, 0, 0, 0, 0};
-        for (int g = 0; g < b.nwg; ++g)
+        for (int g = g0; g < g1; ++g)
+#pragma unroll
             for (int q = 0; q < 5; ++q) s[q] += (double)partials[(int64_t)g * nq + q * b.nbins_pk + i];
+    } else if (i < b.nbins_pk + b.nshell) {
+        for (int g = g0; g < g1; ++g) s[0] += (double)partials[(int64_t)g * nq + 4 * b.nbins_pk + i];
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) red[part][lo][q] = s[q];
+    __syncthreads();
+    if (part != 0) return;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        double r = red[0][lo][q];
+        for (int w = 1; w < 8; ++w) r += red[w][lo][q];
+        s[q] = r;
+    }
+    if (i < b.nbins_pk) {
         double *p = pk_out + 5 * (int64_t)i;
         const double co = s[0];
         p[0] = s[1] / co;
@@ -236,10 +257,7 @@ __global__ void k_bin_finalize(BinPlan b, const float *partials, double kF3, dou
             p[4] = 0.0;
         }
     } else if (i < b.nbins_pk + b.nshell) {
-        const int s = i - b.nbins_pk;
-        double t = 0.0;
-        for (int g = 0; g < b.nwg; ++g) t += (double)partials[(int64_t)g * nq + 5 * b.nbins_pk + s];
-        shell_sum[s] = t;
+        shell_sum[i - b.nbins_pk] = s[0];
     }
 }
 
@@ -310,7 +328,124 @@ __global__ void k_shell_scales(int nshell, const double *shell_sum, float *scale
     scale_exp[s] = e;
 }
 
+
+// ------------------------------------------------------------------ full-array forms
+// fcomb on a full N^3 complex array exactly as powerI4.f90:251-378 receives it (the FFT of
+// grid1 + i grid2), writing the full N^3 output: all eight ordered stores per iteration.
+__global__ void k_fcomb_full_interlaced(int N, float theta, const float *wtab, float cf,
+                                        const float2 *F, float2 *out) {
+    const int h = N / 2 + 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)h * h * h) return;
+    const int ix = (int)(t % h), iy = (int)((t / h) % h), iz = (int)(t / ((int64_t)h * h));
+    const int icx = (N - ix) % N, icy = (N - iy) % N, icz = (N - iz) % N;
+    const int mask = (ix == icx ? 1 : 0) | (iy == icy ? 2 : 0) | (iz == icz ? 4 : 0);
+    auto at = [&](int code) {
+        const int X = (code & 1) ? icx : ix, Y = (code & 2) ? icy : iy, Z = (code & 4) ? icz : iz;
+        return ((int64_t)Z * N + Y) * N + X;
+    };
+    float2 orig[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) orig[k] = F[at(k)];
+    auto cm = [&](int s) {
+        float sn, cs;
+        sincosf(-theta * (float)s, &sn, &cs);
+        return make_float2(-sn, cs);
+    };
+    auto comb = [&](float2 a, float2 b, float2 c) {
+        return cadd(cmul(a, make_float2(1.0f - c.x, -c.y)), cmul(conjf2(b), make_float2(1.0f + c.x, c.y)));
+    };
+    const float cfac = cf / (wtab[ix] * wtab[iy] * wtab[iz]);
+    const float2 c000 = cscale(comb(orig[0], orig[7], cm(ix + iy + iz)), cfac);
+    const float2 c001 = cscale(comb(orig[4], orig[3], cm(ix + iy - iz)), cfac);
+    const float2 c010 = cscale(comb(orig[2], orig[5], cm(ix - iy + iz)), cfac);
+    const float2 c011 = cscale(comb(orig[6], orig[1], cm(ix - iy - iz)), cfac);
+    float2 mem[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mem[k] = make_float2(0.0f, 0.0f);
+    auto put = [&](int code, float2 v) {
+        const int cc = code & ~mask;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j == cc) mem[j] = v;
+    };
+    auto get = [&](int code) {
+        const int cc = code & ~mask;
+        float2 v = mem[0];
+#pragma unroll
+        for (int j = 1; j < 8; ++j)
+            if (j == cc) v = mem[j];
+        return v;
+    };
+    put(0, c000);
+    put(4, c001);
+    put(2, c010);
+    put(6, c011);
+    put(1, conjf2(get(6)));
+    put(5, conjf2(get(2)));
+    put(3, conjf2(get(4)));
+    put(7, conjf2(get(0)));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[at(k)] = get(k);
+}
+
+__global__ void k_fcomb_full_single(int N, const float *wtab, float cf, const float2 *F,
+                                    float2 *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * N * N) return;
+    const int x = (int)(t % N), y = (int)((t / N) % N), z = (int)(t / ((int64_t)N * N));
+    if (x > N / 2) {
+        out[t] = F[t];
+        return;
+    }
+    const int ay = y <= N / 2 ? y : N - y, az = z <= N / 2 ? z : N - z;
+    out[t] = cscale(F[t], cf / (wtab[x] * wtab[ay] * wtab[az]));
+}
+
+// compute_q2k / compute_q4k (powerI4.f90:409-508): on the ix <= N/2+1 half, Q = F k_i k_j / k^2
+// (or k_i k_j k_l k_m / k^4), zero at k = 0; the other half is copied.  kF cancels in the ratio.
+__global__ void k_qk(int N, int norder, int4 idx, const float2 *F, float2 *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * N * N) return;
+    const int x = (int)(t % N), y = (int)((t / N) % N), z = (int)(t / ((int64_t)N * N));
+    if (x > N / 2) {
+        out[t] = F[t];
+        return;
+    }
+    if (x == 0 && y == 0 && z == 0) {
+        out[t] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const float v[3] = {(float)x, (float)(y <= N / 2 ? y : y - N), (float)(z <= N / 2 ? z : z - N)};
+    const float k2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    float f = v[idx.x - 1] * v[idx.y - 1] / k2;
+    if (norder == 4) f = f * (v[idx.z - 1] * v[idx.w - 1] / k2);
+    out[t] = cscale(F[t], f);
+}
+
 }  // namespace
+
+void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
+                       float2 *out, hipStream_t s) {
+    if (interlacing) {
+        const float theta = (float)((double)kTwoPiF32 / 2.0 / (double)N);
+        const int64_t n = (int64_t)(N / 2 + 1) * (N / 2 + 1) * (N / 2 + 1);
+        hipLaunchKernelGGL(k_fcomb_full_interlaced, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                           s, N, theta, wtab, cf, F, out);
+    } else {
+        const int64_t n = (int64_t)N * N * N;
+        hipLaunchKernelGGL(k_fcomb_full_single, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, N,
+                           wtab, cf, F, out);
+    }
+    PBI4_HIP(hipGetLastError());
+}
+
+void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, hipStream_t s) {
+    const int64_t n = (int64_t)N * N * N;
+    const int4 id = make_int4(idx[0], idx[1], norder == 4 ? idx[2] : 1, norder == 4 ? idx[3] : 1);
+    hipLaunchKernelGGL(k_qk, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, N, norder, id, F, out);
+    PBI4_HIP(hipGetLastError());
+}
 
 void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
                     const float2 *G2, float2 *delta, hipStream_t s) {
@@ -345,7 +480,7 @@ void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStrea
 void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, double *pk_out,
                          double *shell_sum, hipStream_t s) {
     const int n = b.nbins_pk + b.nshell;
-    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, 64)), dim3(64), 0, s, b, partials,
+    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, 32)), dim3(256), 0, s, b, partials,
                        kF3, pk_out, shell_sum);
     PBI4_HIP(hipGetLastError());
 }

@@ -149,14 +149,27 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *map
         }
 }
 
-__global__ void k_gemm_reduce(const float *partials, int nchunks, int64_t npair_pad, int ncol_pad,
-                              int npair, int ncol, double *C) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)npair * ncol) return;
-    const int64_t p = t / ncol, c = t % ncol;
+// Split-K combine in FP64: each block owns 32 consecutive C entries; the chunk range of every
+// entry is split over 8 threads in fixed sub-ranges, then summed in a fixed order in LDS.
+__global__ __launch_bounds__(256) void k_gemm_reduce(const float *partials, int nchunks,
+                                                     int64_t npair_pad, int ncol_pad, int npair,
+                                                     int ncol, double *C) {
+    __shared__ double red[8][32];
+    const int lo = threadIdx.x & 31, part = threadIdx.x >> 5;
+    const int64_t t = (int64_t)blockIdx.x * 32 + lo;
+    const bool ok = t < (int64_t)npair * ncol;
+    const int64_t p = ok ? t / ncol : 0, c = ok ? t % ncol : 0;
+    const int per = (nchunks + 7) / 8;
+    const int k0 = part * per, k1 = min(nchunks, k0 + per);
     double s = 0.0;
-    for (int k = 0; k < nchunks; ++k) s += (double)partials[((int64_t)k * npair_pad + p) * ncol_pad + c];
-    C[t] = s;
+    for (int k = k0; k < k1; ++k) s += (double)partials[((int64_t)k * npair_pad + p) * ncol_pad + c];
+    red[part][lo] = s;
+    __syncthreads();
+    if (part == 0 && ok) {
+        double r = red[0][lo];
+        for (int q = 1; q < 8; ++q) r += red[q][lo];
+        C[t] = r;
+    }
 }
 
 // One thread per triangle m (reference order): bout(:, m) of bispectrumI4.f90:173-182 / :489-500
@@ -307,7 +320,7 @@ void launch_gemm(const GemmPlan &g, const float *maps, float *partials, hipStrea
 void launch_gemm_reduce(const GemmPlan &g, const float *partials, double *C, hipStream_t s) {
     const int cf = (int)ceil_div(g.ncol, 16);
     const int64_t n = (int64_t)g.npair * g.ncol;
-    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, partials,
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)ceil_div(n, 32)), dim3(256), 0, s, partials,
                        g.nkchunks, (int64_t)g.nrowblocks * kRowsPerWG, cf * 16, g.npair, g.ncol, C);
     PBI4_HIP(hipGetLastError());
 }

@@ -78,11 +78,13 @@ class CataloguePlan:
         counts = (ntr.astype(np.float64) * float(m.grid) ** 3).astype(np.float32)
         self.counts = torch.from_numpy(counts).to(ctx.device)
         self.desc = {}
+        self.pkdesc = {}
         for f in m.fields:
             self.desc[f] = ctx.bk_desc(nb, m.grid, m.kbin, m.kcenter, m.iopen, m.kf(), self.itot,
                                        1, f.rsd, f.los, m.precision, m.maps_grid)
+            self.pkdesc[f] = ctx.pk_desc(m.grid, m.box, m.kbin, m.kcenter, m.nb_ps(), f.rsd, f.los)
         self.maps_grid = ctx.maps_grid(next(iter(self.desc.values())))
-        self.bk_ws_bytes = max(ctx.bk_workspace(d) for d in self.desc.values())
+        self.bk_ws_bytes = max(ctx.pk_bk_workspace(self.pkdesc[f], self.desc[f]) for f in m.fields)
         self.rsd_coef = rsd_factor(m.z, m.om, m.ol)
 
     def bytes_per_catalogue(self, npart):
@@ -110,13 +112,11 @@ class CataloguePlan:
                                    box=m.box, rsd=rsd, normalised=False, stream=s)
                 delta = ctx.density(grids, m.grid, m.order, m.interlacing, m.box, npart, stream=s)
                 del grids
-                pout = ctx.pk(delta, m.grid, m.box, m.kbin, m.kcenter, m.nb_ps(), f.rsd, f.los,
-                              stream=s)
                 desc = self.desc[f]
                 desc.npart = int(npart)
                 if precision is not None:
                     desc.precision = int(precision)
-                bout = ctx.bk(delta, self.counts, desc, stream=s)
+                pout, bout = ctx.pk_bk(delta, self.counts, self.pkdesc[f], desc, stream=s)
                 desc.precision = int(m.precision)
                 out[f] = (pout, bout)
         return out

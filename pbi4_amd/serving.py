@@ -21,6 +21,23 @@ import torch
 from .pipeline import CataloguePlan, Measurement
 
 
+ADMIT, DEFER, REJECT, WAIT = "admit", "defer", "reject", "wait"
+
+
+def admission(now, arrival, free_slot, need_bytes, free_bytes, in_flight):
+    """Decision for the request at the head of the queue.
+    WAIT: it has not arrived yet; DEFER: no free slot, or not enough free device memory while
+    other requests are in flight (they will release memory); REJECT: it cannot fit even on an
+    idle device; ADMIT otherwise."""
+    if arrival > now:
+        return WAIT
+    if not free_slot:
+        return DEFER
+    if need_bytes > free_bytes:
+        return DEFER if in_flight > 0 else REJECT
+    return ADMIT
+
+
 @dataclass
 class Request:
     rid: int
@@ -102,17 +119,15 @@ class ServingEngine:
         self.finished.extend(done)
         while self.queue:
             r = self.queue[0]
-            if r.arrival > self.now():
-                break
             slot = next((s for s in self.slots if s.req is None), None)
-            if slot is None:
-                break
             plan = self.plan(r.measurement)
             need = plan.bytes_per_catalogue(r.npart) + self.reserve
-            if need > self.free_bytes_fn():
-                if self.in_flight() == 0:
-                    raise MemoryError(f"request {r.rid} needs {need} bytes, more than the device has")
-                self.deferred += 1
+            d = admission(self.now(), r.arrival, slot is not None, need,
+                          self.free_bytes_fn() if slot is not None else 0, self.in_flight())
+            if d == REJECT:
+                raise MemoryError(f"request {r.rid} needs {need} bytes, more than the device has")
+            if d != ADMIT:
+                self.deferred += d == DEFER and slot is not None
                 break
             self.queue.popleft()
             self._launch(slot, r, plan)
@@ -123,6 +138,9 @@ class ServingEngine:
         slot.req = r
         # the request's inputs are produced on the default stream
         slot.stream.wait_stream(torch.cuda.current_stream(self.ctx.device))
+        r.pos.record_stream(slot.stream)
+        if r.vel is not None:
+            r.vel.record_stream(slot.stream)
         slot.outputs = plan.run(r.pos, r.vel, r.npart, stream=slot.stream)
         with torch.cuda.stream(slot.stream):
             for f, (pout, bout) in slot.outputs.items():
