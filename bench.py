@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--roofline-kernel", default="pbi4_deposit")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--stages", type=int, default=1, help="print a per-stage breakdown")
+    ap.add_argument("--gemm", type=int, default=1,
+                    help="time the triangle GEMM alone at the 512^3 / 48-shell shape")
     return ap.parse_args()
 
 
@@ -240,6 +242,9 @@ def main():
         lat["arrival_rate_per_s"] = round(rate, 3)
         log(f"[bench] Poisson stream at {rate:.1f}/s: {lat}")
 
+    # ---------------------------------------------------------------- triangle GEMM alone
+    gemm = gemm_bench(ctx, torch) if args.gemm else None
+
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if args.cpu_baseline:
@@ -264,9 +269,49 @@ def main():
         "split_bf16_vs_fp32_err": None if errs is None else
         {"max": errs[0], "median": errs[1], "p99": errs[2]},
         "stage_ms_rsd_field": stages,
+        "triangle_gemm": gemm,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def gemm_bench(ctx, torch):
+    """MFMA utilisation of the split-BF16x3 triangle GEMM at the precision-config shape: 48 shells
+    (kbin = kcenter = 1 at grid 512, max |q| = 48) on the band-limited maps grid 150^3, real space
+    (48 columns) and redshift space (144 columns); random unit-variance maps."""
+    from pbi4_amd import _lib
+    out = {}
+    nb, Np = 48, 150
+    K = Np ** 3
+    Ks = ((K + 63) // 64) * 64
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    for nell in (1, 3):
+        maps = torch.randn((nb * nell, Ks), generator=g, device="cuda", dtype=torch.float32)
+        maps[:, K:] = 0
+        npair = nb * (nb + 1) // 2
+        res = {}
+        for prec, name, peak in ((_lib.PREC_BF16X3, "bf16x3", BF16_PEAK_TFLOPS),
+                                 (_lib.PREC_FP32, "fp32", 157.3)):
+            ctx.triangle_sums(maps, nb, nell, K, prec)
+            torch.cuda.synchronize()
+            ctx.timing_enable("pbi4_triangle_gemm")
+            reps = 5
+            for _ in range(reps):
+                ctx.triangle_sums(maps, nb, nell, K, prec)
+            torch.cuda.synchronize()
+            ms, n = ctx.timing_read()
+            ctx.timing_enable(None)
+            avg = ms / max(n, 1)
+            logical = 2.0 * npair * nb * nell * K
+            executed = logical * (3 if prec == _lib.PREC_BF16X3 else 1)
+            tf = executed / (avg * 1e-3) / 1e12
+            res[name] = {"avg_ms": round(avg, 4), "executed_tflops": round(tf, 1),
+                         "mfma_frac": round(tf / peak, 4), "logical_tflops": round(logical / (avg * 1e-3) / 1e12, 1)}
+        out[f"nb48_nell{nell}_K150^3"] = res
+        del maps
+    log(f"[bench] triangle GEMM alone: {out}")
+    return out
 
 
 def cpu_baseline(cat, plan, m, N, box, ctx):

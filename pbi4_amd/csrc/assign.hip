@@ -66,8 +66,15 @@ __device__ __forceinline__ float norm_pos(const PosXform &x, const float *pos, c
     if (x.pos_mode == PBI4_POS_NORMALISED) return p;
     float s = p;
     if (x.has_vel && x.rsd[d] != 0.0f) s = fmaf(x.rsd[d], vel[(int64_t)d * npart + i], s);
-    float t = fmodf(s + x.box, x.box);
-    if (t < 0.0f) t += x.box;
+    // (s + box) % box: for s + box in [0, 3 box) subtracting box is exact (Sterbenz), so this
+    // equals fmod there; fmodf covers anything further out
+    float t = s + x.box;
+    if (t >= x.box) t -= x.box;
+    if (t >= x.box) t -= x.box;
+    if (t < 0.0f || t >= x.box) {
+        t = fmodf(t, x.box);
+        if (t < 0.0f) t += x.box;
+    }
     float q = t / x.box;
     return fminf(q, 1.0f);
 }
@@ -308,6 +315,16 @@ __device__ __forceinline__ float deposit_scale(int single) {
     return (ORDER >= 3) ? 1048576.0f : (ORDER == 2 && single) ? 262144.0f : 524288.0f;
 }
 
+// Lanes per particle: a group of SIDE x SIDE lanes handles one particle, lane (a, b) owning the
+// x-offset a and y-offset b of its stencil and looping over z; the SIDE^2 addresses a group
+// touches in one instruction (a 4 x 4 block in an x-y plane, row stride R = T + 4) fall in
+// distinct LDS banks, and the particles sharing a wave are nw apart in tile order.
+template <int ORDER>
+struct DepositGroup {
+    static constexpr int SIDE = ORDER >= 3 ? 4 : ORDER;
+    static constexpr int G = SIDE * SIDE;
+};
+
 template <int ORDER>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int ngrids,
                                                              const uint32_t *offsets,
@@ -315,6 +332,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int n
                                                              const uint32_t *sy,
                                                              const uint32_t *sz,
                                                              float *regions) {
+    constexpr int SIDE = DepositGroup<ORDER>::SIDE, G = DepositGroup<ORDER>::G;
     __shared__ int acc[2 * kMaxR * kMaxR * kMaxR];
     const int R3 = R * R * R, ncell = ngrids * R3;
     const int single = ngrids == 1;
@@ -324,42 +342,38 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int n
     const int64_t tile = blockIdx.x;
     float *out = regions + tile * (int64_t)ncell;
     const uint32_t b = offsets[tile], e = offsets[tile + 1];
-    const uint32_t round = (uint32_t)blockDim.x * per;
+    const uint32_t round = (uint32_t)blockDim.x * per;      // particles per LDS round
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int gl = lane % G, gi = lane / G;
+    const int la = gl % SIDE, lb = gl / SIDE;
+    const bool active = la < ORDER && lb < ORDER;
+    const uint32_t per_step = blockDim.x / G;
     bool first = true;
     uint32_t r0 = b;
     do {
         for (int i = threadIdx.x; i < ncell; i += blockDim.x) acc[i] = 0;
         __syncthreads();
-        // lane-major swizzle: the 64 lanes of a wave take particles 16 apart in tile order, so
-        // neighbours in the sort (often neighbours in space) land in different waves and their
-        // stencils rarely hit the same LDS address within one atomic instruction
-        const uint32_t nw = blockDim.x >> 6;
-        const uint32_t sw = (threadIdx.x & 63) * nw + (threadIdx.x >> 6);
-        for (int j = 0; j < per; ++j) {
-            const uint32_t p = r0 + sw + (uint32_t)j * blockDim.x;
-            if (p >= e) continue;
-            const uint32_t q[3] = {sx[p], sy[p], sz[p]};
-            int c[3];
-            float f[3];
-            for (int d = 0; d < 3; ++d) {
-                c[d] = (int)(q[d] >> 28);
-                f[d] = (float)(q[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
-            }
-            for (int g = 0; g < ngrids; ++g) {
-                int st[3];
-                float w[3][4];
-                for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
-                int *a = acc + g * R3;
-#pragma unroll 1
-                for (int iz = 0; iz < ORDER; ++iz) {
+        const uint32_t rend = min(e, r0 + round);
+        for (uint32_t s0 = r0; s0 < rend; s0 += per_step) {
+            const uint32_t p = s0 + (uint32_t)(gi * nw + wave);
+            if (p < rend && active) {
+                const uint32_t q[3] = {sx[p], sy[p], sz[p]};
+                int c[3];
+                float f[3];
+                for (int d = 0; d < 3; ++d) {
+                    c[d] = (int)(q[d] >> 28);
+                    f[d] = (float)(q[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
+                }
+                for (int g = 0; g < ngrids; ++g) {
+                    int st[3];
+                    float w[3][4];
+                    for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
+                    const float wxy = w[0][la] * w[1][lb];
+                    int *a = acc + g * R3 + ((st[2] + kHalo) * R + st[1] + kHalo + lb) * R + st[0] + kHalo + la;
 #pragma unroll
-                    for (int iy = 0; iy < ORDER; ++iy) {
-                        const float wyz = w[1][iy] * w[2][iz];
-                        const int row = ((st[2] + kHalo + iz) * R + st[1] + kHalo + iy) * R + st[0] + kHalo;
-#pragma unroll
-                        for (int ix = 0; ix < ORDER; ++ix)
-                            atomicAdd(&a[row + ix], __float2int_rn((w[0][ix] * wyz) * fix));
-                    }
+                    for (int iz = 0; iz < ORDER; ++iz)
+                        atomicAdd(a + iz * R * R, __float2int_rn((wxy * w[2][iz]) * fix));
                 }
             }
         }
@@ -370,7 +384,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int n
             for (int i = threadIdx.x; i < ncell; i += blockDim.x) out[i] += (float)acc[i] * inv;
         }
         first = false;
-        r0 += round;
+        r0 = rend;
         __syncthreads();
     } while (r0 < e);
 }
@@ -406,6 +420,7 @@ __global__ __launch_bounds__(256) void k_gather(int N, int T, int NT, int R, int
     }
     const int64_t V = (int64_t)N * N * N;
     float *out = grids + (int64_t)g * V + ((int64_t)(ty * T + cy) * N + tx * T + cx);
+#pragma unroll 4
     for (int cz = 0; cz < T; ++cz) {
         int ntz[2], niz[2], nz = 1;
         ntz[0] = tz;

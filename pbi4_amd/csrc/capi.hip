@@ -42,6 +42,7 @@ namespace {
 
 // P(k) bin of (|kx|, |ky|, |kz|): imk = nint(rk/dk - kcenter/kbin - tiny) + 1, with
 // kF = REAL(4) twopi/box and float(ik) (powerI4.f90:527-566).  0 = outside 1..nbins.
+// Stored [|kz|][|ky|][kx] so the binning kernel's lanes (consecutive kx) read contiguously.
 std::vector<uint16_t> build_pk_table(int N, float box, float kbin, float kcenter, int nbins) {
     const int h = N / 2 + 1;
     const double kF = (double)(kTwoPiF32 / box);
@@ -58,7 +59,7 @@ std::vector<uint16_t> build_pk_table(int N, float box, float kbin, float kcenter
             for (int c = 0; c < h; ++c) {
                 const double rk = std::sqrt(sq[a] + sq[b] + sq[c]);
                 const long imk = std::lround(rk / dk - kcb - DBL_MIN) + 1;
-                t[((size_t)a * h + b) * h + c] = (imk >= 1 && imk <= nbins) ? (uint16_t)imk : 0;
+                t[((size_t)c * h + b) * h + a] = (imk >= 1 && imk <= nbins) ? (uint16_t)imk : 0;
             }
     return t;
 }

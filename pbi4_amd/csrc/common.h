@@ -85,7 +85,7 @@ struct BinPlan {
     int multipoles;
     float los[3];
     float kF;                 // |k| scale for avgk
-    const uint16_t *pk_table; // (N/2+1)^3 bin ids, 0 = excluded
+    const uint16_t *pk_table; // (N/2+1)^3 bin ids [|kz|][|ky|][kx], 0 = excluded
     const float2 *cross;      // second field of a cross spectrum (real space), or null
     const float2 *f2, *f4;    // explicit delta_2, delta_4 (multipoles), or null for L_l(mu)
     const int16_t *shell_table; // d2 -> shell id, 0 = none, size max_d2+1

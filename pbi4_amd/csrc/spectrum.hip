@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
                 float v[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 if (valid) {
                     const bool sel = x != 0 || ky > 0 || (ky == 0 && kz > 0);
-                    const int bin = sel ? (int)b.pk_table[((int64_t)x * h + aky) * h + akz] : 0;
+                    const int bin = sel ? (int)b.pk_table[((int64_t)akz * h + aky) * h + x] : 0;
                     key = bin;
                     if (bin > 0) {
                         const float k2 = (float)(x * x + aky * aky + akz * akz);
