@@ -2,11 +2,12 @@
 //
 // Particles are first binned by tile (T^3 cells, T <= 16) with a counting sort, then one
 // work-group per tile deposits its particles into LDS copies of the tile plus a 2-cell halo on
-// each side ((T+4)^3 cells per grid), accumulating in 64-bit fixed point (2^-32 units) so the
-// result does not depend on the order of the LDS atomics.  Each tile writes its halo-extended
-// block to a region buffer with plain coalesced stores; a gather pass then sums, for every
-// grid cell, the (1 to 8) region entries that cover it in a fixed order.  No global atomics
-// touch grid memory, and the output is bitwise reproducible.
+// each side ((T+4)^3 cells per grid), accumulating in 32-bit fixed point (2^-18 .. 2^-20 units,
+// chosen per tile so no cell can overflow) so the result does not depend on the order of the
+// LDS atomics.  Each tile writes its halo-extended block to a region buffer in pieces grouped by
+// the tile that owns the cells; a gather pass then sums, for every grid cell, the (1 to 8)
+// region entries that cover it in a fixed order.  No global atomics touch grid memory, and the
+// output is bitwise reproducible.
 //
 // Stencils relative to the cell c = floor(u) containing the particle (u = N * pos), written
 // with the fractional part f = u - c, follow the reference exactly:
@@ -26,9 +27,8 @@ constexpr int kHalo = 2;          // region cells below the tile origin
 constexpr int kMaxR = 16 + 4;     // T <= 16
 constexpr int kDepositThreads = 1024;
 constexpr int kSortThreads = 1024;
+constexpr int kHistPer = 8;       // particles per thread per histogram pass (loads in flight)
 constexpr int64_t kLdsHistMax = 32768;   // tiles counted in LDS (u32) per work-group
-constexpr float kFix = 4294967296.0f;    // 2^32 fixed-point scale
-constexpr float kInvFix = 2.3283064365386963e-10f;
 
 struct PosXform {
     int N, pos_mode;
@@ -58,14 +58,31 @@ __device__ __forceinline__ void grid_coord(const PosXform &x, float p, int &c, f
     c = ci;
 }
 
+// One particle's raw components.  Loads are unconditional (the caller clamps the index into
+// range; a component without RSD reads its position twice) so that the loads of several
+// particles issue as one batch; guarded loads become branches with a wait after each.
+struct RawParticle {
+    float p[3], v[3];
+};
+
+__device__ __forceinline__ RawParticle load_particle(const PosXform &x, const float *pos,
+                                                     const float *vel, int64_t i, int64_t npart) {
+    RawParticle r;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float *vd = (x.has_vel && x.rsd[d] != 0.0f) ? vel : pos;
+        r.p[d] = pos[(int64_t)d * npart + i];
+        r.v[d] = vd[(int64_t)d * npart + i];
+    }
+    return r;
+}
+
 // Normalised position component as the reference computes it: (s + box) % box / box in
 // float32 after the RSD shift (pbi4.py:154, 242-243), or the given value if already in [0,1).
-__device__ __forceinline__ float norm_pos(const PosXform &x, const float *pos, const float *vel,
-                                          int d, int64_t i, int64_t npart) {
-    float p = pos[(int64_t)d * npart + i];
+__device__ __forceinline__ float norm_pos(const PosXform &x, const RawParticle &r, int d) {
+    const float p = r.p[d];
     if (x.pos_mode == PBI4_POS_NORMALISED) return p;
-    float s = p;
-    if (x.has_vel && x.rsd[d] != 0.0f) s = fmaf(x.rsd[d], vel[(int64_t)d * npart + i], s);
+    const float s = (x.has_vel && x.rsd[d] != 0.0f) ? fmaf(x.rsd[d], r.v[d], p) : p;
     // (s + box) % box: for s + box in [0, 3 box) subtracting box is exact (Sterbenz), so this
     // equals fmod there; fmodf covers anything further out
     float t = s + x.box;
@@ -75,17 +92,18 @@ __device__ __forceinline__ float norm_pos(const PosXform &x, const float *pos, c
         t = fmodf(t, x.box);
         if (t < 0.0f) t += x.box;
     }
-    float q = t / x.box;
+    const float q = t / x.box;
     return fminf(q, 1.0f);
 }
 
-__device__ __forceinline__ int tile_of(const PosXform &x, int T, int NT, const float *pos,
-                                       const float *vel, int64_t i, int64_t npart,
+__device__ __forceinline__ int tile_of(const PosXform &x, int T, int NT, const RawParticle &r,
                                        uint32_t *packed) {
     int c[3];
     float f[3];
-    for (int d = 0; d < 3; ++d) grid_coord(x, norm_pos(x, pos, vel, d, i, npart), c[d], f[d]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) grid_coord(x, norm_pos(x, r, d), c[d], f[d]);
     int t[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         t[d] = c[d] / T;
         if (packed) {
@@ -137,19 +155,32 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, i
     if (use_lds) {
         for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
         __syncthreads();
-        for (int64_t i0 = beg; i0 < end; i0 += blockDim.x) {
-            const int64_t i = i0 + threadIdx.x;
-            const int t = i < end ? tile_of(x, T, NT, pos, vel, i, npart, nullptr) : -1;
-            int rank, len;
-            run_of(t, rank, len);
-            if (t >= 0 && rank == 0) atomicAdd(&lhist[t], (uint32_t)len);
+        for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * kHistPer) {
+            RawParticle r[kHistPer];
+#pragma unroll
+            for (int k = 0; k < kHistPer; ++k) {
+                const int64_t i = i0 + threadIdx.x + (int64_t)k * blockDim.x;
+                r[k] = load_particle(x, pos, vel, i < end ? i : beg, npart);
+            }
+            int t[kHistPer];
+#pragma unroll
+            for (int k = 0; k < kHistPer; ++k) {
+                const int64_t i = i0 + threadIdx.x + (int64_t)k * blockDim.x;
+                t[k] = i < end ? tile_of(x, T, NT, r[k], nullptr) : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kHistPer; ++k) {
+                int rank, len;
+                run_of(t[k], rank, len);
+                if (t[k] >= 0 && rank == 0) atomicAdd(&lhist[t[k]], (uint32_t)len);
+            }
         }
         __syncthreads();
         for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x)
             if (lhist[t]) atomicAdd(&counts[t], lhist[t]);
     } else {
         for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
-            atomicAdd(&counts[tile_of(x, T, NT, pos, vel, i, npart, nullptr)], 1u);
+            atomicAdd(&counts[tile_of(x, T, NT, load_particle(x, pos, vel, i, npart), nullptr)], 1u);
     }
 }
 
@@ -193,7 +224,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
     if (!use_lds) {
         for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
             uint32_t pk[3];
-            int t = tile_of(x, T, NT, pos, vel, i, npart, pk);
+            int t = tile_of(x, T, NT, load_particle(x, pos, vel, i, npart), pk);
             uint32_t slot = atomicAdd(&cursor[t], 1u);
             sx[slot] = pk[0];
             sy[slot] = pk[1];
@@ -208,10 +239,16 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
         uint32_t pk[kScatterPer][3];
         int tl[kScatterPer];
         uint32_t rank[kScatterPer];
+        RawParticle raw[kScatterPer];
+#pragma unroll
+        for (int k = 0; k < kScatterPer; ++k) {
+            const int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
+            raw[k] = load_particle(x, pos, vel, i < end ? i : beg, npart);
+        }
 #pragma unroll
         for (int k = 0; k < kScatterPer; ++k) {
             int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
-            tl[k] = i < end ? tile_of(x, T, NT, pos, vel, i, npart, pk[k]) : -1;
+            tl[k] = i < end ? tile_of(x, T, NT, raw[k], pk[k]) : -1;
             int rr, len;
             run_of(tl[k], rr, len);
             uint32_t base = 0;
@@ -238,6 +275,27 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
 }
 
 // ------------------------------------------------------------------ deposit
+
+// Region layout.  A tile deposits into the (T+4)^3 cells [-2, T+2)^3 around it (region
+// coordinates a = cell - origin + 2).  Along each axis a region coordinate has a class: 0 for
+// a < 2 (cells of the previous tile, its local T-2, T-1), 1 for the T home cells, 2 for
+// a >= T+2 (the next tile's local 0, 1).  The region is stored as 27 pieces, one per class
+// triple, in (cz, cy, cx) order, each piece z-y-x row-major.  Every piece is read by exactly one
+// gather work-group (the tile that owns its cells) as one contiguous block.
+__device__ __forceinline__ int region_class(int a, int T) { return a < kHalo ? 0 : a < T + kHalo ? 1 : 2; }
+__device__ __forceinline__ int class_start(int c, int T) { return c == 0 ? 0 : c == 1 ? kHalo : T + kHalo; }
+__device__ __forceinline__ int class_extent(int c, int T) { return c == 1 ? T : kHalo; }
+
+__device__ __forceinline__ int region_index(int a, int b, int c, int T, int R) {
+    const int ka = region_class(a, T), kb = region_class(b, T), kc = region_class(c, T);
+    const int sa = class_start(ka, T), sb = class_start(kb, T), sc = class_start(kc, T);
+    const int ea = class_extent(ka, T), eb = class_extent(kb, T), ec = class_extent(kc, T);
+    // pieces before (kc, kb, ka): all of the lower z classes, then the lower y classes of this z
+    // class, then the lower x classes of this (z, y) class
+    const int off = sc * R * R + ec * (sb * R + eb * sa);
+    return off + ((c - sc) * eb + (b - sb)) * ea + (a - sa);
+}
+
 
 // Per-dimension stencil start (relative to the tile origin, before the halo offset) and
 // weights, for one grid of the given order (see the table at the top of this file).
@@ -299,31 +357,124 @@ __device__ __forceinline__ void stencil(int grid_sel, int single, int c, float f
     }
 }
 
-// LDS accumulation is 32-bit fixed point.  A round deposits up to kPer(ORDER) particles per
-// thread; the scale 2^-s is chosen so that no LDS cell can overflow within a round: the largest
-// single contribution is (2/3)^3 for PCS, (3/4)^3 for TSC, 1 for CIC / NGP and 1.5^3 for the
-// single-grid CIC of powerI4.f90:114-124 (h = vr - nint(vr) gives weights in [-0.5, 1.5]).
-// After each round the LDS tile is written to (first round) or added to (later rounds, fixed
-// order) the tile's region; integer sums within a round are order independent, so the grids
-// are bitwise reproducible.
+// LDS accumulation is 32-bit fixed point.  A chunk of kPer(ORDER) particles per thread at scale
+// 2^-s cannot overflow an LDS cell: the largest single contribution is (2/3)^3 for PCS, (3/4)^3
+// for TSC, 1 for CIC / NGP and 1.5^3 for the single-grid CIC of powerI4.f90:114-124
+// (h = vr - nint(vr) gives weights in [-0.5, 1.5]).  A tile of n particles deposits 2^k chunks
+// per round at scale 2^-(s-k), k = ceil(log2(n / chunk)), which keeps the same bound: dense tiles
+// use one round instead of many (each extra round re-reads and re-writes the region).  k stops
+// where the scale would fall below 2^-kMinScaleBits, which bounds the rounding error of one
+// contribution by 2^-19.  After each round the LDS tile is written to (first round) or added to
+// (later rounds, fixed order) the tile's region; integer sums within a round are order
+// independent, so the grids are bitwise reproducible.
+constexpr int kMinScaleBits = 18;
 template <int ORDER>
 __device__ __forceinline__ int deposit_per_thread(int single) {
     return (ORDER >= 3) ? 4 : (ORDER == 2 && single) ? 1 : 2;
 }
 template <int ORDER>
-__device__ __forceinline__ float deposit_scale(int single) {
-    return (ORDER >= 3) ? 1048576.0f : (ORDER == 2 && single) ? 262144.0f : 524288.0f;
+__device__ __forceinline__ int deposit_scale_bits(int single) {
+    return (ORDER >= 3) ? 20 : (ORDER == 2 && single) ? 18 : 19;
 }
 
-// Lanes per particle: a group of SIDE x SIDE lanes handles one particle, lane (a, b) owning the
-// x-offset a and y-offset b of its stencil and looping over z; the SIDE^2 addresses a group
-// touches in one instruction (a 4 x 4 block in an x-y plane, row stride R = T + 4) fall in
-// distinct LDS banks, and the particles sharing a wave are nw apart in tile order.
+// Lane layout of the deposit: X lanes share one particle, lane ix owning the stencil's x-offset
+// ix and looping over (y, z); a wave deposits P = 64 / X particles per atomic instruction.  The
+// wave's particles are preloaded with coalesced loads (lane L holds particles L + 64 m of the
+// wave's chunk) and handed out by shuffles so that the P particles of one step are X apart in
+// tile order: for particles that arrive in spatial order their x-footprints tile a row instead of
+// overlapping, and a cell crowded with particles does not serialise one instruction.
 template <int ORDER>
-struct DepositGroup {
-    static constexpr int SIDE = ORDER >= 3 ? 4 : ORDER;
-    static constexpr int G = SIDE * SIDE;
+struct DepositLanes {
+    static constexpr int X = ORDER == 3 ? 4 : ORDER;
+    static constexpr int P = 64 / X;
 };
+constexpr int kDepositPerMax = 4;
+
+// w[i] for a lane-dependent i, as bit masks: a select of array elements can otherwise be turned
+// into a dynamically indexed (scratch) load
+template <int ORDER>
+__device__ __forceinline__ float pick4(const float w[4], int i) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < (ORDER == 3 ? 4 : ORDER); ++k) r |= __float_as_uint(w[k]) & (0u - (uint32_t)(i == k));
+    return __uint_as_float(r);
+}
+
+// Position in the (T+4)^3 region of LDS cell l (l = (c R + b) R + a) and of l + blockDim.x.
+struct RegionCursor {
+    int g, a, b, c;
+    __device__ void init(int l, int R) {
+        const int R2 = R * R, R3 = R2 * R;
+        g = l / R3;
+        l -= g * R3;
+        c = l / R2;
+        l -= c * R2;
+        b = l / R;
+        a = l - b * R;
+    }
+    __device__ void advance(int da, int db, int dc, int R) {
+        a += da;
+        if (a >= R) { a -= R; ++b; }
+        b += db;
+        if (b >= R) { b -= R; ++c; }
+        c += dc;
+        while (c >= R) { c -= R; ++g; }
+    }
+};
+
+// Deposits particles [c0, cend) of a tile into the LDS accumulators (one preload per thread).
+template <int ORDER>
+__device__ __forceinline__ void deposit_chunk(int *acc, int R, int R3, int ngrids, int single,
+                                              int per, float fix, uint32_t c0, uint32_t cend,
+                                              const uint32_t *sx, const uint32_t *sy,
+                                              const uint32_t *sz) {
+    constexpr int X = DepositLanes<ORDER>::X;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ix = lane % X, j = lane / X;
+    const uint32_t cb = c0 + (uint32_t)wave * 64u * (uint32_t)per;   // this wave's part
+    uint32_t q[kDepositPerMax][3];
+#pragma unroll
+    for (int m = 0; m < kDepositPerMax; ++m) {
+        const uint32_t p = cb + (uint32_t)(lane + 64 * m);
+        const bool ok = m < per && p < cend;
+        q[m][0] = ok ? sx[p] : 0u;
+        q[m][1] = ok ? sy[p] : 0u;
+        q[m][2] = ok ? sz[p] : 0u;
+    }
+    __syncthreads();   // the accumulators are cleared before the first chunk of a round
+#pragma unroll
+    for (int m = 0; m < kDepositPerMax; ++m) {
+        if (m >= per || cb + 64u * m >= cend) continue;
+#pragma unroll 1
+        for (int s = 0; s < X; ++s) {
+            const int src = j * X + s;
+            uint32_t qq[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) qq[d] = (uint32_t)__shfl((int)q[m][d], src);
+            if (cb + (uint32_t)(64 * m + src) >= cend || ix >= ORDER) continue;
+            int c[3];
+            float f[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                c[d] = (int)(qq[d] >> 28);
+                f[d] = (float)(qq[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
+            }
+            for (int g = 0; g < ngrids; ++g) {
+                int st[3];
+                float w[3][4];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
+                const float wx = pick4<ORDER>(w[0], ix) * fix;
+                int *a = acc + g * R3 + ((st[2] + kHalo) * R + st[1] + kHalo) * R + st[0] + kHalo + ix;
+#pragma unroll
+                for (int iz = 0; iz < ORDER; ++iz)
+#pragma unroll
+                    for (int iy = 0; iy < ORDER; ++iy)
+                        atomicAdd(a + (iz * R + iy) * R, __float2int_rn(wx * (w[1][iy] * w[2][iz])));
+            }
+        }
+    }
+}
 
 template <int ORDER>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int ngrids,
@@ -332,56 +483,38 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int n
                                                              const uint32_t *sy,
                                                              const uint32_t *sz,
                                                              float *regions) {
-    constexpr int SIDE = DepositGroup<ORDER>::SIDE, G = DepositGroup<ORDER>::G;
     __shared__ int acc[2 * kMaxR * kMaxR * kMaxR];
     const int R3 = R * R * R, ncell = ngrids * R3;
     const int single = ngrids == 1;
     const int per = deposit_per_thread<ORDER>(single);
-    const float fix = deposit_scale<ORDER>(single);
-    const float inv = 1.0f / fix;
     const int64_t tile = blockIdx.x;
     float *out = regions + tile * (int64_t)ncell;
     const uint32_t b = offsets[tile], e = offsets[tile + 1];
-    const uint32_t round = (uint32_t)blockDim.x * per;      // particles per LDS round
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    const int gl = lane % G, gi = lane / G;
-    const int la = gl % SIDE, lb = gl / SIDE;
-    const bool active = la < ORDER && lb < ORDER;
-    const uint32_t per_step = blockDim.x / G;
+    const uint32_t chunk = (uint32_t)blockDim.x * per;      // particles per preload
+    const int bits = deposit_scale_bits<ORDER>(single);
+    int drop = 0;
+    while (bits - drop > kMinScaleBits && (chunk << drop) < e - b) ++drop;
+    const uint32_t round = chunk << drop;                    // particles per LDS round
+    const float fix = (float)(1 << (bits - drop));
+    const float inv = 1.0f / fix;
+    // flush cursor: this thread's first LDS cell and the stride blockDim.x in (a, b, c)
+    const int sc = blockDim.x / (R * R), sb = (blockDim.x / R) % R, sa = blockDim.x % R;
     bool first = true;
     uint32_t r0 = b;
     do {
         for (int i = threadIdx.x; i < ncell; i += blockDim.x) acc[i] = 0;
-        __syncthreads();
         const uint32_t rend = min(e, r0 + round);
-        for (uint32_t s0 = r0; s0 < rend; s0 += per_step) {
-            const uint32_t p = s0 + (uint32_t)(gi * nw + wave);
-            if (p < rend && active) {
-                const uint32_t q[3] = {sx[p], sy[p], sz[p]};
-                int c[3];
-                float f[3];
-                for (int d = 0; d < 3; ++d) {
-                    c[d] = (int)(q[d] >> 28);
-                    f[d] = (float)(q[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
-                }
-                for (int g = 0; g < ngrids; ++g) {
-                    int st[3];
-                    float w[3][4];
-                    for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
-                    const float wxy = w[0][la] * w[1][lb];
-                    int *a = acc + g * R3 + ((st[2] + kHalo) * R + st[1] + kHalo + lb) * R + st[0] + kHalo + la;
-#pragma unroll
-                    for (int iz = 0; iz < ORDER; ++iz)
-                        atomicAdd(a + iz * R * R, __float2int_rn((wxy * w[2][iz]) * fix));
-                }
-            }
-        }
+        for (uint32_t c0 = r0; c0 < rend; c0 += chunk)
+            deposit_chunk<ORDER>(acc, R, R3, ngrids, single, per, fix, c0, min(rend, c0 + chunk),
+                                 sx, sy, sz);
         __syncthreads();
-        if (first) {
-            for (int i = threadIdx.x; i < ncell; i += blockDim.x) out[i] = (float)acc[i] * inv;
-        } else {
-            for (int i = threadIdx.x; i < ncell; i += blockDim.x) out[i] += (float)acc[i] * inv;
+        RegionCursor rc;
+        rc.init(threadIdx.x, R);
+        for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+            float *o = out + rc.g * R3 + region_index(rc.a, rc.b, rc.c, T, R);
+            const float v = (float)acc[i] * inv;
+            *o = first ? v : *o + v;
+            rc.advance(sa, sb, sc, R);
         }
         first = false;
         r0 = rend;
@@ -389,58 +522,62 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int n
     } while (r0 < e);
 }
 
-// grid[g][z][y][x] = sum of the region entries covering (x,y,z) in a fixed order (home tile,
-// then the previous / next tile along each axis).  One work-group per (tile, grid): thread
-// (x, y) of the T x T face walks z.
-__global__ __launch_bounds__(256) void k_gather(int N, int T, int NT, int R, int ngrids,
-                                                const float *regions, float *grids) {
+// grid[g][z][y][x] = sum of the region entries covering (x,y,z), in a fixed order: for each
+// (x, y) option (home first) its home-z entry plus its z-neighbour's entry.  One work-group per
+// (tile, grid); thread (x, y) of the T x T face holds its T cells along z in registers and issues
+// each contribution's loads as one independent batch.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+void k_gather(int N, int T, int NT, int R, int ngrids, const float *regions, float *grids) {
     const int tile = blockIdx.x, g = blockIdx.y;
     const int tx = tile % NT, ty = (tile / NT) % NT, tz = tile / (NT * NT);
     const int cx = threadIdx.x % 16, cy = threadIdx.x / 16;
     if (cx >= T || cy >= T) return;
     const int R3 = R * R * R;
-    int ntx[2], nix[2], nx = 1, nty[2], niy[2], ny = 1;
-    ntx[0] = tx;
-    nix[0] = cx + kHalo;
-    if (cx < kHalo) {
-        ntx[nx] = (tx + NT - 1) % NT;
-        nix[nx++] = cx + T + kHalo;
-    } else if (cx >= T - kHalo) {
-        ntx[nx] = (tx + 1) % NT;
-        nix[nx++] = cx - T + kHalo;
-    }
-    nty[0] = ty;
-    niy[0] = cy + kHalo;
-    if (cy < kHalo) {
-        nty[ny] = (ty + NT - 1) % NT;
-        niy[ny++] = cy + T + kHalo;
-    } else if (cy >= T - kHalo) {
-        nty[ny] = (ty + 1) % NT;
-        niy[ny++] = cy - T + kHalo;
+    // along each axis: option 0 = home (region coordinate cell + 2), option 1 = the neighbour
+    // whose halo covers the cell (previous tile for cells < 2, next tile for cells >= T - 2)
+    int nx = 1, ny = 1;
+    int ntx[2] = {tx, tx}, nix[2] = {cx + kHalo, 0};
+    int nty[2] = {ty, ty}, niy[2] = {cy + kHalo, 0};
+    if (cx < kHalo) { ntx[1] = (tx + NT - 1) % NT; nix[1] = cx + T + kHalo; nx = 2; }
+    else if (cx >= T - kHalo) { ntx[1] = (tx + 1) % NT; nix[1] = cx - T + kHalo; nx = 2; }
+    if (cy < kHalo) { nty[1] = (ty + NT - 1) % NT; niy[1] = cy + T + kHalo; ny = 2; }
+    else if (cy >= T - kHalo) { nty[1] = (ty + 1) % NT; niy[1] = cy - T + kHalo; ny = 2; }
+    const int tzm = (tz + NT - 1) % NT, tzp = (tz + 1) % NT;
+    float s[16];
+    // cells beyond T re-read cell T-1 and are never stored: unconditional loads stay batched
+    // (a guarded load becomes a branch with a wait after it)
+    for (int o = 0; o < nx * ny; ++o) {
+        const int ox = o % nx, oy = o / nx;
+        const int64_t tb = ((int64_t)tz * NT + nty[oy]) * NT + ntx[ox];
+        const int64_t tm = ((int64_t)tzm * NT + nty[oy]) * NT + ntx[ox];
+        const int64_t tp = ((int64_t)tzp * NT + nty[oy]) * NT + ntx[ox];
+        const float *r = regions + (tb * ngrids + g) * R3;
+        const float *rm = regions + (tm * ngrids + g) * R3;
+        const float *rp = regions + (tp * ngrids + g) * R3;
+        float v[16], lo[2], hi[2];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = r[region_index(nix[ox], niy[oy], min(k, T - 1) + kHalo, T, R)];
+        // z-neighbours of the same (x, y) option: cells 0, 1 from the previous z tile, cells
+        // T-2, T-1 from the next one
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            lo[k] = rm[region_index(nix[ox], niy[oy], k + T + kHalo, T, R)];
+            hi[k] = rp[region_index(nix[ox], niy[oy], k, T, R)];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            float t = v[k];
+            if (k < 2) t += lo[k < 2 ? k : 0];
+            if (k == T - 2) t += hi[0];
+            if (k == T - 1) t += hi[1];
+            s[k] = o == 0 ? t : s[k] + t;
+        }
     }
     const int64_t V = (int64_t)N * N * N;
-    float *out = grids + (int64_t)g * V + ((int64_t)(ty * T + cy) * N + tx * T + cx);
-#pragma unroll 4
-    for (int cz = 0; cz < T; ++cz) {
-        int ntz[2], niz[2], nz = 1;
-        ntz[0] = tz;
-        niz[0] = cz + kHalo;
-        if (cz < kHalo) {
-            ntz[nz] = (tz + NT - 1) % NT;
-            niz[nz++] = cz + T + kHalo;
-        } else if (cz >= T - kHalo) {
-            ntz[nz] = (tz + 1) % NT;
-            niz[nz++] = cz - T + kHalo;
-        }
-        float s = 0.0f;
-        for (int a = 0; a < nz; ++a)
-            for (int bb = 0; bb < ny; ++bb)
-                for (int c = 0; c < nx; ++c) {
-                    const int64_t t = ((int64_t)ntz[a] * NT + nty[bb]) * NT + ntx[c];
-                    s += regions[(t * ngrids + g) * R3 + (niz[a] * R + niy[bb]) * R + nix[c]];
-                }
-        out[(int64_t)(tz * T + cz) * N * N] = s;
-    }
+    float *out = grids + (int64_t)g * V + ((int64_t)(tz * T) * N + ty * T + cy) * N + tx * T + cx;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        if (k < T) out[(int64_t)k * N * N] = s[k];
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {

@@ -115,8 +115,9 @@ struct GemmPlan {
     int nb, nell, ncol, npair;
     int64_t K, Kstride; // map length and per-map stride (floats)
     int precision;
+    int cf, rf;         // column fragments (of 16), row fragments per wave (of 16)
     int kchunk;         // x-values per work-group
-    int nkchunks, nrowblocks;
+    int nkchunks, nkchunks_pad, nrowblocks;
 };
 GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision);
 size_t gemm_partials_floats(const GemmPlan &g);

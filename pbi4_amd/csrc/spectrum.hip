@@ -112,31 +112,99 @@ __global__ void k_combine_single(int N, const float *wtab, float cf, const float
 constexpr int kBinThreads = 256;
 constexpr int kBinWaves = kBinThreads / 64;
 
-// Segmented suffix sum across a wave for runs of equal key (keys non-decreasing in lane order);
-// afterwards the first lane of each run holds the run total.
-template <int NV>
-__device__ __forceinline__ void seg_suffix_sum(int key, float (&v)[NV], int lane) {
+// DPP lane moves (VALU, no LDS): CTRL 0x110 + n = row_shr:n inside 16-lane rows, 0x142 =
+// row_bcast:15 (lane 15 of each row to the next row), 0x143 = row_bcast:31.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false); }
+
+template <int CTRL, int NV>
+__device__ __forceinline__ void seg_step(int key, float (&v)[NV], bool ok) {
+    const int k2 = dpp_i<CTRL>(key);
+    float t[NV];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int k2 = __shfl_down(key, off);
-        float t[NV];
+    for (int i = 0; i < NV; ++i) t[i] = __int_as_float(dpp_i<CTRL>(__float_as_int(v[i])));
+    if (ok && k2 == key) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) t[i] = __shfl_down(v[i], off);
-        if (lane + off < 64 && k2 == key) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) v[i] += t[i];
-        }
+        for (int i = 0; i < NV; ++i) v[i] += t[i];
     }
 }
 
-// One wave per (y, z) row of the half spectrum, 64 consecutive x per step.  Along a row |k|
+// Segmented inclusive prefix sum across a wave for runs of equal key, keys non-decreasing in
+// lane order; afterwards the last lane of each run holds the run total.  A source lane with the
+// same key implies every lane between has it, so each step adds exactly the run's lanes.
+template <int NV>
+__device__ __forceinline__ void seg_prefix_sum(int key, float (&v)[NV], int lane) {
+    const int rl = lane & 15;
+    seg_step<0x111>(key, v, rl >= 1);
+    seg_step<0x112>(key, v, rl >= 2);
+    seg_step<0x114>(key, v, rl >= 4);
+    seg_step<0x118>(key, v, rl >= 8);
+    seg_step<0x142>(key, v, (lane & 31) >= 16);
+    seg_step<0x143>(key, v, lane >= 32);
+}
+
+// Contributions of one mode (x, y, z) of the half spectrum: P(k) bin key with (count, k, P0, P2,
+// P4) and shell key with its weighted power.
+struct ModeTerms {
+    int key, skey;
+    float v[5], sv[1];
+};
+
+__device__ __forceinline__ ModeTerms mode_terms(const BinPlan &b, const float2 *delta, int64_t row,
+                                                int N, int h, int x, int y, int ky, int aky,
+                                                int z, int kz, int akz, float lt, bool valid) {
+    ModeTerms t;
+    t.key = -1;
+    t.skey = -1;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) t.v[q] = 0.0f;
+    t.sv[0] = 0.0f;
+    if (!valid) return t;
+    const float2 d = delta[row * h + x];
+    const float p = d.x * d.x + d.y * d.y;
+    const int d2 = x * x + aky * aky + akz * akz;
+    if (b.nbins_pk > 0) {
+        const bool sel = x != 0 || ky > 0 || (ky == 0 && kz > 0);
+        const int bin = sel ? (int)b.pk_table[((int64_t)akz * h + aky) * h + x] : 0;
+        t.key = bin;
+        if (bin > 0) {
+            auto dot = [&](const float2 *f) {   // Re(conj(d) * f) at this mode
+                const float2 e = f[row * h + x];
+                return d.x * e.x + d.y * e.y;
+            };
+            const float k2 = (float)d2;
+            const float lk = b.los[0] * (float)x + lt;
+            const float mu2 = k2 > 0.0f ? lk * lk / k2 : 0.0f;
+            t.v[0] = 1.0f;
+            t.v[1] = b.kF * sqrtf(k2);
+            t.v[2] = (!b.multipoles && b.cross) ? dot(b.cross) : p;
+            if (b.multipoles) {
+                t.v[3] = b.f2 ? dot(b.f2) : (-0.5f + 1.5f * mu2) * p;
+                t.v[4] = b.f4 ? dot(b.f4) : (0.375f + mu2 * (-3.75f + 4.375f * mu2)) * p;
+            }
+        }
+    }
+    if (b.nshell > 0) {
+        const int s = d2 <= b.max_d2 ? (int)b.shell_table[d2] : 0;
+        t.skey = s;
+        if (s > 0) {
+            const bool planex = x == 0 || x == N / 2;
+            const bool special = planex && (y == 0 || y == N / 2) && (z == 0 || z == N / 2);
+            t.sv[0] = (special ? d.x * d.x : p) * (planex ? 1.0f : 2.0f);
+        }
+    }
+    return t;
+}
+
+// One wave per (y, z) row of the half spectrum, 64 consecutive x per step over x < N/2; the
+// lane holding x = N/2 - 1 also adds the Nyquist mode x = N/2 after the scan.  Along a row |k|
 // grows with x, so P(k) bins and shell ids are non-decreasing in lane order and each run is
-// reduced in registers; the run heads add into wave-private LDS accumulators (no atomics), the
-// waves are summed in a fixed order and each work-group writes one partial: deterministic.
+// reduced in registers; run tails add into wave-private LDS accumulators (no atomics), the waves
+// are summed in a fixed order and each work-group writes one partial: deterministic.
 __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *delta,
                                                      float *partials) {
     extern __shared__ float acc[];
-    const int N = b.N, h = N / 2 + 1;
+    const int N = b.N, h = N / 2 + 1, half = N / 2;
     const int nq = 5 * b.nbins_pk + b.nshell;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kBinWaves * nq; i += blockDim.x) acc[i] = 0.0f;
@@ -149,59 +217,31 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
         const int ky = y <= N / 2 ? y : y - N, kz = z <= N / 2 ? z : z - N;
         const int aky = ky < 0 ? -ky : ky, akz = kz < 0 ? -kz : kz;
         const float lt = b.los[1] * (float)ky + b.los[2] * (float)kz;
-        for (int x0 = 0; x0 < h; x0 += 64) {
+        for (int x0 = 0; x0 < half; x0 += 64) {
             const int x = x0 + lane;
-            const bool valid = x < h;
-            float2 d = make_float2(0.0f, 0.0f);
-            if (valid) d = delta[row * h + x];
-            const float p = d.x * d.x + d.y * d.y;
-            auto dot = [&](const float2 *f) {   // Re(conj(d) * f) at this mode
-                const float2 e = f[row * h + x];
-                return d.x * e.x + d.y * e.y;
-            };
+            ModeTerms t = mode_terms(b, delta, row, N, h, x, y, ky, aky, z, kz, akz, lt, x < half);
+            const bool nyq_lane = x == half - 1;
+            ModeTerms tn;
+            if (nyq_lane) tn = mode_terms(b, delta, row, N, h, half, y, ky, aky, z, kz, akz, lt, true);
+            const int last = min(63, half - 1 - x0);      // highest valid lane of this step
             if (b.nbins_pk > 0) {
-                int key = -1;
-                float v[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                if (valid) {
-                    const bool sel = x != 0 || ky > 0 || (ky == 0 && kz > 0);
-                    const int bin = sel ? (int)b.pk_table[((int64_t)akz * h + aky) * h + x] : 0;
-                    key = bin;
-                    if (bin > 0) {
-                        const float k2 = (float)(x * x + aky * aky + akz * akz);
-                        const float lk = b.los[0] * (float)x + lt;
-                        const float mu2 = k2 > 0.0f ? lk * lk / k2 : 0.0f;
-                        v[0] = 1.0f;
-                        v[1] = b.kF * sqrtf(k2);
-                        v[2] = (!b.multipoles && b.cross) ? dot(b.cross) : p;
-                        if (b.multipoles) {
-                            v[3] = b.f2 ? dot(b.f2) : (-0.5f + 1.5f * mu2) * p;
-                            v[4] = b.f4 ? dot(b.f4) : (0.375f + mu2 * (-3.75f + 4.375f * mu2)) * p;
-                        }
-                    }
-                }
-                seg_suffix_sum<5>(key, v, lane);
-                const int prev = __shfl_up(key, 1);
-                if (key > 0 && (lane == 0 || prev != key)) {
+                seg_prefix_sum<5>(t.key, t.v, lane);
+                const int next = __shfl_down(t.key, 1);
+                if (t.key > 0 && (lane == last || next != t.key)) {
 #pragma unroll
-                    for (int q = 0; q < 5; ++q) mine[q * b.nbins_pk + key - 1] += v[q];
+                    for (int q = 0; q < 5; ++q) mine[q * b.nbins_pk + t.key - 1] += t.v[q];
+                }
+                if (nyq_lane && tn.key > 0) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) mine[q * b.nbins_pk + tn.key - 1] += tn.v[q];
                 }
             }
             if (b.nshell > 0) {
-                int key = -1;
-                float v[1] = {0.0f};
-                if (valid) {
-                    const int d2 = x * x + aky * aky + akz * akz;
-                    const int s = d2 <= b.max_d2 ? (int)b.shell_table[d2] : 0;
-                    key = s;
-                    if (s > 0) {
-                        const bool planex = x == 0 || x == N / 2;
-                        const bool special = planex && (y == 0 || y == N / 2) && (z == 0 || z == N / 2);
-                        v[0] = (special ? d.x * d.x : p) * (planex ? 1.0f : 2.0f);
-                    }
-                }
-                seg_suffix_sum<1>(key, v, lane);
-                const int prev = __shfl_up(key, 1);
-                if (key > 0 && (lane == 0 || prev != key)) mine[5 * b.nbins_pk + key - 1] += v[0];
+                seg_prefix_sum<1>(t.skey, t.sv, lane);
+                const int next = __shfl_down(t.skey, 1);
+                if (t.skey > 0 && (lane == last || next != t.skey))
+                    mine[5 * b.nbins_pk + t.skey - 1] += t.sv[0];
+                if (nyq_lane && tn.skey > 0) mine[5 * b.nbins_pk + tn.skey - 1] += tn.sv[0];
             }
         }
     }
