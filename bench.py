@@ -49,7 +49,23 @@ def parse():
     ap.add_argument("--stages", type=int, default=1, help="print a per-stage breakdown")
     ap.add_argument("--gemm", type=int, default=1,
                     help="time the triangle GEMM alone at the 512^3 / 48-shell shape")
+    ap.add_argument("--gemm-only", type=int, default=0,
+                    help="run only the triangle GEMM timing (for profiling), print it and exit")
     return ap.parse_args()
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` (calibrated FETCH_SIZE + WRITE_SIZE) from the rocprofv3
+    summary committed in profiles/ for this workload, or None if it is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_256cube_pcs.txt")
+    try:
+        for line in open(path):
+            f = line.split()
+            if f and f[0] == kernel and len(f) >= 3:
+                return round((float(f[1]) + float(f[2])) * 1e6)
+    except OSError:
+        pass
+    return None
 
 
 def bf16_error(ref_out, out):
@@ -87,6 +103,9 @@ def main():
 
     torch.cuda.init()
     ctx = Context(0)
+    if args.gemm_only:
+        print(json.dumps({"triangle_gemm": gemm_bench(ctx, torch)}), flush=True)
+        return 0
     N, n = args.grid, args.lattice
     box = 1000.0
     m = Measurement(grid=N, order=4, interlacing=True, box=box, kbin=2.0, kcenter=2.0,
@@ -161,7 +180,8 @@ def main():
         roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
         achieved = float("nan")
     roof.update({"kernel": args.roofline_kernel, "achieved": round(achieved, 2),
-                 "frac": round(achieved / roof["peak"], 4), "traffic": None,
+                 "frac": round(achieved / roof["peak"], 4),
+                 "traffic": pmc_traffic("k_deposit<4>") if args.roofline_kernel == "pbi4_deposit" else None,
                  "avg_launch_ms": round(avg_ms, 5), "launches": launches,
                  "algorithmic_bytes_per_launch": alg_bytes})
 

@@ -19,13 +19,12 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGemmThreads = 256;     // 4 waves
+constexpr int kPadF = 4;              // float padding per LDS row
+constexpr int kPadB = 8;              // bf16 padding per LDS row
 constexpr int kMaxCF = 12;            // column fragments (ncol <= 192)
 constexpr int kXcds = 8;              // MI355X: work-groups are dealt round-robin to 8 XCDs
-constexpr int kKT = 32;               // x-values per LDS stage: 8 quads (128 B) per map row
-constexpr int kGemmLds = 80 * 1024;   // LDS ring per work-group (two work-groups per CU)
 
-typedef __attribute__((address_space(3))) void lds_void_t;
-typedef __attribute__((address_space(1))) void gbl_void_t;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void pair_of(int p, int &i, int &j) {
     // p = j(j+1)/2 + i with 0 <= i <= j
@@ -36,96 +35,45 @@ __device__ __forceinline__ void pair_of(int p, int &i, int &j) {
     i = p - jj * (jj + 1) / 2;
 }
 
-// Ring geometry: each stage holds NRP map rows x kKT floats; a wave's load instruction fills
-// 8 rows (64 lanes x 16 B), so NRP is the column count rounded up to 32 (4 waves).
 template <int CF>
-struct GemmRing {
-    static constexpr int NRP = (CF * 16 + 31) / 32 * 32;
-    static constexpr int LPW = NRP / 32;                        // load instructions per wave
-    static constexpr int STAGE = NRP * kKT;                     // floats
-    static constexpr int S0 = kGemmLds / (STAGE * 4);
-    static constexpr int S = S0 > 8 ? 8 : S0;                   // stages in the ring
+struct GemmTile {
+    static constexpr int KT = CF <= 6 ? 64 : 32;                 // x-values per LDS stage
+    static constexpr int NV = (CF * 16 * KT / 4 + kGemmThreads - 1) / kGemmThreads;  // float4/thread
 };
 
-// Wait until at most n of this wave's stage loads are outstanding, then barrier.  The loads
-// write LDS directly (no registers), so the compiler does not track them: the counts are ours,
-// and the memory clobber keeps LDS reads below the barrier.
-template <int LPW>
-__device__ __forceinline__ void wait_stages_barrier(int n) {
-#define PBI4_WAIT(N)                                                         \
-    case N:                                                                  \
-        if constexpr (N * LPW <= 63)                                         \
-            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(N * LPW) : "memory"); \
-        break;
-    switch (n) {
-        PBI4_WAIT(0)
-        PBI4_WAIT(1)
-        PBI4_WAIT(2)
-        PBI4_WAIT(3)
-        PBI4_WAIT(4)
-        PBI4_WAIT(5)
-        PBI4_WAIT(6)
-        default: asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); break;
-    }
-#undef PBI4_WAIT
-}
-
-// Quad q (4 floats) of ring row r is stored at quad position q ^ (r & 7): the 16 rows one MFMA
-// fragment reads at one quad fall in distinct LDS banks.
-__device__ __forceinline__ float4 ring_quad(const float *stage, int r, int q) {
-    return *reinterpret_cast<const float4 *>(stage + r * kKT + 4 * (q ^ (r & 7)));
-}
-
-__device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const __bf16 h = (__bf16)v[e];
-        hi[e] = h;
-        lo[e] = (__bf16)(v[e] - (float)h);
-    }
+template <int PREC>
+__host__ __device__ constexpr size_t gemm_stage_bytes(int nb, int ncol_pad, int kt) {
+    return PREC == PBI4_PREC_BF16X3
+               ? sizeof(float) * nb * (kt + kPadF) + 2 * sizeof(__bf16) * ncol_pad * (kt + kPadB)
+               : sizeof(float) * ncol_pad * (kt + kPadF);
 }
 
 // Work-group tile: 4 waves x RF row fragments x 16 pair-rows, all CF column fragments, one K
-// chunk.  The chunk streams through an S-stage LDS ring filled by direct global->LDS loads
-// (S-1 stages in flight, one barrier per stage); A = M_i M_j is formed from the ring's ell = 0
-// rows and split into bf16 hi/lo in registers, as is B.  x beyond the chunk end loads clamped
-// (finite) data and its A products are zeroed.  The linear work-group id is mapped so that all
-// row blocks of one K chunk land on the same XCD (ids = chunk mod 8): the chunk's maps come from
-// HBM once and from that XCD's L2 for the other row blocks.
+// chunk.  LDS is double buffered: the global loads of stage s+1 are in flight while stage s is
+// multiplied; B is split into bf16 hi/lo once, when stored to LDS.  The linear work-group id is
+// mapped so that all row blocks of one K chunk land on the same XCD (ids = chunk mod 8): the
+// chunk's maps come from HBM once and from that XCD's L2 for the other row blocks.
 template <int PREC, int CF, int RF>
-__global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *maps, int64_t K, int64_t Kstride, int nb, int ncol, int npair,
-                     int kchunk, int nrowblocks, int nchunks, float *partials) {
-    using G = GemmRing<CF>;
-    constexpr int S = G::S, LPW = G::LPW, STAGE = G::STAGE;
+__global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *maps, int64_t K,
+                                                                int64_t Kstride, int nb, int ncol,
+                                                                int npair, int kchunk,
+                                                                int nrowblocks, int nchunks,
+                                                                float *partials) {
+    constexpr int KT = GemmTile<CF>::KT, NV = GemmTile<CF>::NV;
+    constexpr int LDF = KT + kPadF, LDB = KT + kPadB;
     constexpr int NCP = CF * 16;
-    extern __shared__ __align__(16) float ring[];
+    extern __shared__ __align__(16) unsigned char smem[];
+    const size_t stage = gemm_stage_bytes<PREC>(nb, NCP, KT);
 
     const int w = blockIdx.x;
-    const int qd = w / kXcds;
-    const int rowblock = qd % nrowblocks;
-    const int chunk = (w % kXcds) + kXcds * (qd / nrowblocks);
+    const int q = w / kXcds;
+    const int rowblock = q % nrowblocks;
+    const int chunk = (w % kXcds) + kXcds * (q / nrowblocks);
     if (chunk >= nchunks) return;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane >> 4;
     const int64_t kbeg = (int64_t)chunk * kchunk;
     const int64_t kend = min(K, kbeg + kchunk);
-    const int nst = (int)((kend - kbeg + kKT - 1) / kKT);
-
-    // loader: instruction i = wave + 4 j fills ring rows 8 i .. 8 i + 7; lane L writes row
-    // 8 i + L / 8 = row0 + 32 j at quad position L % 8, i.e. quad (L % 8) ^ (row0 & 7) of that
-    // map row (rows past the last map re-read it; the ring never reads them)
-    const int row0 = 8 * wave + lane / 8;
-    const int lq = (lane % 8) ^ (row0 & 7);
-    auto issue = [&](int st) {
-        const int64_t x = min(kbeg + (int64_t)st * kKT + 4 * lq, K - 4);
-        float *dst = ring + (st % S) * STAGE;
-#pragma unroll
-        for (int j = 0; j < LPW; ++j) {
-            const float *src = maps + (int64_t)min(row0 + 32 * j, ncol - 1) * Kstride + x;
-            __builtin_amdgcn_global_load_lds((gbl_void_t *)src,
-                                             (lds_void_t *)(dst + (wave + 4 * j) * 256), 16, 0, 0);
-        }
-    };
 
     int pi[RF], pj[RF];
 #pragma unroll
@@ -140,64 +88,99 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *map
 #pragma unroll
         for (int c = 0; c < CF; ++c) acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    auto compute = [&](int st) {
-        const float *b = ring + (st % S) * STAGE;
-        // x-values of this stage still inside the chunk, relative to the stage start
-        const int64_t left = kend - (kbeg + (int64_t)st * kKT);
+    float4 reg[NV];
+    auto load = [&](int64_t x0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int e = threadIdx.x + v * kGemmThreads;
+            const int c = e / (KT / 4), xv = e % (KT / 4);
+            const int64_t x = x0 + 4 * xv;
+            reg[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (c < ncol && x < kend) reg[v] = *reinterpret_cast<const float4 *>(maps + (int64_t)c * Kstride + x);
+        }
+    };
+    auto store = [&](int buf) {
+        unsigned char *base = smem + buf * stage;
+        float *tf = reinterpret_cast<float *>(base);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int e = threadIdx.x + v * kGemmThreads;
+            const int c = e / (KT / 4), xv = e % (KT / 4);
+            if (c >= NCP) continue;
+            if (PREC == PBI4_PREC_BF16X3) {
+                if (c < nb) *reinterpret_cast<float4 *>(tf + c * LDF + 4 * xv) = reg[v];
+                __bf16 *bhi = reinterpret_cast<__bf16 *>(base + sizeof(float) * nb * LDF);
+                __bf16 *blo = bhi + NCP * LDB;
+                const float vv[4] = {reg[v].x, reg[v].y, reg[v].z, reg[v].w};
+                bf16x4 h, l;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const __bf16 hb = (__bf16)vv[k];
+                    h[k] = hb;
+                    l[k] = (__bf16)(vv[k] - (float)hb);
+                }
+                *reinterpret_cast<bf16x4 *>(bhi + c * LDB + 4 * xv) = h;
+                *reinterpret_cast<bf16x4 *>(blo + c * LDB + 4 * xv) = l;
+            } else {
+                *reinterpret_cast<float4 *>(tf + c * LDF + 4 * xv) = reg[v];
+            }
+        }
+    };
+    auto compute = [&](int buf) {
+        const unsigned char *base = smem + buf * stage;
+        const float *tf = reinterpret_cast<const float *>(base);
         if (PREC == PBI4_PREC_BF16X3) {
-            // lane: k = 8 grp .. 8 grp + 7 = quads 2 grp, 2 grp + 1
-            const int q0 = 2 * grp;
-            bf16x8 ahi[RF], alo[RF];
+            const __bf16 *bhi = reinterpret_cast<const __bf16 *>(base + sizeof(float) * nb * LDF);
+            const __bf16 *blo = bhi + NCP * LDB;
 #pragma unroll
-            for (int r = 0; r < RF; ++r) {
-                const float4 a0 = ring_quad(b, pi[r], q0), a1 = ring_quad(b, pi[r], q0 + 1);
-                const float4 c0 = ring_quad(b, pj[r], q0), c1 = ring_quad(b, pj[r], q0 + 1);
-                float prod[8] = {a0.x * c0.x, a0.y * c0.y, a0.z * c0.z, a0.w * c0.w,
-                                 a1.x * c1.x, a1.y * c1.y, a1.z * c1.z, a1.w * c1.w};
-                if (left < kKT) {
-#pragma unroll
-                    for (int e = 0; e < 8; ++e)
-                        if (8 * grp + e >= left) prod[e] = 0.0f;
-                }
-                split8(prod, ahi[r], alo[r]);
-            }
-#pragma unroll
-            for (int c = 0; c < CF; ++c) {
-                const int col = c * 16 + (lane & 15);
-                const float4 b0 = ring_quad(b, col, q0), b1 = ring_quad(b, col, q0 + 1);
-                const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-                bf16x8 bh, bl;
-                split8(bv, bh, bl);
+            for (int ks = 0; ks < KT; ks += 32) {
+                const int xo = ks + 8 * grp;
+                bf16x8 ahi[RF], alo[RF];
 #pragma unroll
                 for (int r = 0; r < RF; ++r) {
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r], bh, acc[r][c], 0, 0, 0);
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r], bl, acc[r][c], 0, 0, 0);
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r], bh, acc[r][c], 0, 0, 0);
-                }
-            }
-        } else {
-            // two k-steps of 16: lane supplies k = 4 grp + e of each MFMA e
+                    const float4 *ai = reinterpret_cast<const float4 *>(tf + pi[r] * LDF + xo);
+                    const float4 *aj = reinterpret_cast<const float4 *>(tf + pj[r] * LDF + xo);
+                    const float4 a0 = ai[0], a1 = ai[1], b0 = aj[0], b1 = aj[1];
+                    const float prod[8] = {a0.x * b0.x, a0.y * b0.y, a0.z * b0.z, a0.w * b0.w,
+                                           a1.x * b1.x, a1.y * b1.y, a1.z * b1.z, a1.w * b1.w};
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                const int q = 4 * ks + grp;
-                float pr[RF][4];
-#pragma unroll
-                for (int r = 0; r < RF; ++r) {
-                    const float4 a = ring_quad(b, pi[r], q), c4 = ring_quad(b, pj[r], q);
-                    pr[r][0] = a.x * c4.x;
-                    pr[r][1] = a.y * c4.y;
-                    pr[r][2] = a.z * c4.z;
-                    pr[r][3] = a.w * c4.w;
-                    if (left < kKT) {
-#pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            if (4 * q + e >= left) pr[r][e] = 0.0f;
+                    for (int e = 0; e < 8; ++e) {
+                        const __bf16 h = (__bf16)prod[e];
+                        ahi[r][e] = h;
+                        alo[r][e] = (__bf16)(prod[e] - (float)h);
                     }
                 }
 #pragma unroll
                 for (int c = 0; c < CF; ++c) {
                     const int col = c * 16 + (lane & 15);
-                    const float4 bv = ring_quad(b, col, q);
+                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(bhi + col * LDB + xo);
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(blo + col * LDB + xo);
+#pragma unroll
+                    for (int r = 0; r < RF; ++r) {
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r], bh, acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r], bl, acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r], bh, acc[r][c], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KT; ks += 16) {
+                const int xo = ks + 4 * grp;
+                float pr[RF][4];
+#pragma unroll
+                for (int r = 0; r < RF; ++r) {
+                    const float4 a = *reinterpret_cast<const float4 *>(tf + pi[r] * LDF + xo);
+                    const float4 b = *reinterpret_cast<const float4 *>(tf + pj[r] * LDF + xo);
+                    pr[r][0] = a.x * b.x;
+                    pr[r][1] = a.y * b.y;
+                    pr[r][2] = a.z * b.z;
+                    pr[r][3] = a.w * b.w;
+                }
+#pragma unroll
+                for (int c = 0; c < CF; ++c) {
+                    const int col = c * 16 + (lane & 15);
+                    const float4 bv = *reinterpret_cast<const float4 *>(tf + col * LDF + xo);
 #pragma unroll
                     for (int r = 0; r < RF; ++r) {
                         acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[r][0], bv.x, acc[r][c], 0, 0, 0);
@@ -210,14 +193,19 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *map
         }
     };
 
-    for (int t = 0; t < S - 1; ++t)
-        if (t < nst) issue(t);
-    for (int st = 0; st < nst; ++st) {
-        // stages issued after st: st+1 .. min(nst-1, st+S-2); they may stay in flight
-        wait_stages_barrier<LPW>(min(S - 2, nst - 1 - st));
-        // every wave is past compute(st - 1): its buffer is free for stage st + S - 1
-        if (st + S - 1 < nst) issue(st + S - 1);
-        compute(st);
+    // padded columns (ncol <= c < NCP) are zero in both buffers for the whole kernel: the loads
+    // give zeros there and store() writes them
+    int buf = 0;
+    load(kbeg);
+    store(0);
+    __syncthreads();
+    for (int64_t x0 = kbeg; x0 < kend; x0 += KT) {
+        const bool more = x0 + KT < kend;
+        if (more) load(x0 + KT);
+        compute(buf);
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
     // D layout of the 16x16 MFMAs: lane holds rows 4*(lane/16) + i, column lane%16
     const int64_t npair_pad = (int64_t)nrowblocks * 4 * RF * 16;
@@ -361,8 +349,9 @@ GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precis
     g.rf = g.npair >= 900 ? 4 : g.npair >= 300 ? 2 : 1;
     g.nrowblocks = (int)ceil_div(g.npair, 64 * g.rf);
     // ~2048 work-groups, K chunks a multiple of the LDS stage
+    const int kt = g.cf <= 6 ? 64 : 32;
     int64_t target = std::max<int64_t>(kXcds, 2048 / g.nrowblocks);
-    int64_t kc = std::max<int64_t>(kKT * 8, align_up(ceil_div(K, target), kKT));
+    int64_t kc = std::max<int64_t>(kt * 4, align_up(ceil_div(K, target), kt));
     g.kchunk = (int)kc;
     g.nkchunks = (int)ceil_div(K, kc);
     g.nkchunks_pad = (int)align_up(g.nkchunks, kXcds);
@@ -375,8 +364,9 @@ size_t gemm_partials_floats(const GemmPlan &g) {
 
 template <int PREC, int CF, int RF>
 static void launch_one(const GemmPlan &g, const float *maps, float *partials, hipStream_t s) {
-    using G = GemmRing<CF>;
-    const size_t lds = sizeof(float) * G::S * G::STAGE;
+    constexpr int KT = GemmTile<CF>::KT;
+    const size_t lds = 2 * gemm_stage_bytes<PREC>(g.nb, CF * 16, KT);
+    PBI4_CHECK(lds <= 160 * 1024, PBI4_EUNSUPPORTED, "triangle GEMM tile does not fit in LDS");
     auto kern = k_triangle_gemm<PREC, CF, RF>;
     if (lds > 64 * 1024)
         PBI4_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
