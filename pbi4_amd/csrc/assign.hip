@@ -53,8 +53,13 @@ __device__ __forceinline__ void grid_coord(const PosXform &x, float p, int &c, f
         f -= 1.0f;
     }
     if (f >= 1.0f) f = 0.99999994f;
-    ci %= x.N;
-    if (ci < 0) ci += x.N;
+    // ci is in [-1, N] for components in [0, 1]; the modulo only serves inputs outside
+    if (ci >= x.N) ci -= x.N;
+    else if (ci < 0) ci += x.N;
+    if ((unsigned)ci >= (unsigned)x.N) {
+        ci %= x.N;
+        if (ci < 0) ci += x.N;
+    }
     c = ci;
 }
 
@@ -103,9 +108,14 @@ __device__ __forceinline__ int tile_of(const PosXform &x, int T, int NT, const R
 #pragma unroll
     for (int d = 0; d < 3; ++d) grid_coord(x, norm_pos(x, r, d), c[d], f[d]);
     int t[3];
+    const float invT = 1.0f / (float)T;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        t[d] = c[d] / T;
+        // c / T for 0 <= c < 2^23 from the float reciprocal, corrected by one either way
+        int q = (int)((float)c[d] * invT);
+        if (q * T > c[d]) --q;
+        else if ((q + 1) * T <= c[d]) ++q;
+        t[d] = q;
         if (packed) {
             uint32_t loc = (uint32_t)(c[d] - t[d] * T);
             uint32_t fr = (uint32_t)(f[d] * 268435456.0f);   // 2^28
@@ -118,30 +128,37 @@ __device__ __forceinline__ int tile_of(const PosXform &x, int T, int NT, const R
 
 // ------------------------------------------------------------------ counting sort
 
+// DPP lane moves of a 64-lane wave (gfx9 encodings): 0x110 + n row_shr:n (lane i <- i - n inside
+// a 16-lane row), 0x142 / 0x143 row_bcast:15 / :31, 0x138 wave_shr:1 (i <- i - 1), 0x130
+// wave_shl:1 (i <- i + 1).  VALU only, no LDS traffic.
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+
 // Runs of equal keys among consecutive lanes of a wave (catalogues stored in spatial order put
-// neighbouring particles in the same tile): rank = position in the run, len = run length at the
-// run's first lane.  One LDS atomic per run instead of one per lane.
-__device__ __forceinline__ void run_of(int key, int &rank, int &len) {
-    const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(key, 1);
+// neighbouring particles in the same tile): rank = position in the run, head / tail = first /
+// last lane of the run.  The head below each lane is an inclusive max-scan of head positions.
+// One LDS atomic per run instead of one per lane.
+__device__ __forceinline__ void run_rank(int key, int &rank, bool &tail) {
+    const int lane = threadIdx.x & 63, rl = lane & 15;
+    const int prev = dpp_mov<0x138>(key), next = dpp_mov<0x130>(key);
     const bool head = lane == 0 || prev != key;
-    // index of the run head at or below this lane: inclusive max-scan of head positions
-    int h = head ? lane : 0;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(h, off);
-        if (lane >= off) h = max(h, o);
-    }
+    tail = lane == 63 || next != key;
+    int h = head ? lane : 0, t;
+    t = dpp_mov<0x111>(h);
+    if (rl >= 1) h = max(h, t);
+    t = dpp_mov<0x112>(h);
+    if (rl >= 2) h = max(h, t);
+    t = dpp_mov<0x114>(h);
+    if (rl >= 4) h = max(h, t);
+    t = dpp_mov<0x118>(h);
+    if (rl >= 8) h = max(h, t);
+    t = dpp_mov<0x142>(h);
+    if ((lane & 31) >= 16) h = max(h, t);
+    t = dpp_mov<0x143>(h);
+    if (lane >= 32) h = max(h, t);
     rank = lane - h;
-    // run length: next head above (or 64) minus this head, valid at the head
-    int nh = head ? lane : 64;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_down(nh, off);
-        if (lane + off < 64) nh = min(nh, o);
-    }
-    const int next = __shfl_down(nh, 1);
-    len = (lane + 1 < 64 ? next : 64) - lane;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, int NT,
@@ -170,9 +187,10 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, i
             }
 #pragma unroll
             for (int k = 0; k < kHistPer; ++k) {
-                int rank, len;
-                run_of(t[k], rank, len);
-                if (t[k] >= 0 && rank == 0) atomicAdd(&lhist[t[k]], (uint32_t)len);
+                int rank;
+                bool tail;
+                run_rank(t[k], rank, tail);
+                if (t[k] >= 0 && tail) atomicAdd(&lhist[t[k]], (uint32_t)(rank + 1));
             }
         }
         __syncthreads();
@@ -249,11 +267,17 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
         for (int k = 0; k < kScatterPer; ++k) {
             int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
             tl[k] = i < end ? tile_of(x, T, NT, raw[k], pk[k]) : -1;
-            int rr, len;
-            run_of(tl[k], rr, len);
+            int rr;
+            bool tail;
+            run_rank(tl[k], rr, tail);
+            const int lane = threadIdx.x & 63;
             uint32_t base = 0;
-            if (tl[k] >= 0 && rr == 0) base = atomicAdd(&lhist[tl[k]], (uint32_t)len);
-            rank[k] = __shfl(base, (threadIdx.x & 63) - rr) + (uint32_t)rr;
+            if (tl[k] >= 0 && tail) base = atomicAdd(&lhist[tl[k]], (uint32_t)(rr + 1));
+            // the tail hands the run's base to the head (non-tails push to lane + 1, never a
+            // head, so every head receives exactly one value); members read it from the head
+            const int dst = tail ? lane - rr : lane + 1;
+            const uint32_t at_head = (uint32_t)__builtin_amdgcn_ds_permute(4 * dst, (int)base);
+            rank[k] = (uint32_t)__shfl((int)at_head, lane - rr) + (uint32_t)rr;
         }
         __syncthreads();
         for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
@@ -265,6 +289,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
         for (int k = 0; k < kScatterPer; ++k) {
             if (tl[k] >= 0) {
                 uint32_t slot = lhist[tl[k]] + rank[k];
+                if (slot >= (uint32_t)npart) continue;   // cannot happen; keeps a bad rank in bounds
                 sx[slot] = pk[k][0];
                 sy[slot] = pk[k][1];
                 sz[slot] = pk[k][2];
@@ -423,11 +448,11 @@ struct RegionCursor {
 };
 
 // Deposits particles [c0, cend) of a tile into the LDS accumulators (one preload per thread).
-template <int ORDER>
-__device__ __forceinline__ void deposit_chunk(int *acc, int R, int R3, int ngrids, int single,
-                                              int per, float fix, uint32_t c0, uint32_t cend,
-                                              const uint32_t *sx, const uint32_t *sy,
-                                              const uint32_t *sz) {
+template <int ORDER, int NG>
+__device__ __forceinline__ void deposit_chunk(int *acc, int R, int R3, int per, float fix,
+                                              uint32_t c0, uint32_t cend, const uint32_t *sx,
+                                              const uint32_t *sy, const uint32_t *sz) {
+    constexpr int single = NG == 1;
     constexpr int X = DepositLanes<ORDER>::X;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ix = lane % X, j = lane / X;
@@ -459,7 +484,8 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R, int R3, int ngrid
                 c[d] = (int)(qq[d] >> 28);
                 f[d] = (float)(qq[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
             }
-            for (int g = 0; g < ngrids; ++g) {
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
                 int st[3];
                 float w[3][4];
 #pragma unroll
@@ -476,16 +502,16 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R, int R3, int ngrid
     }
 }
 
-template <int ORDER>
-__global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int ngrids,
+template <int ORDER, int NG>
+__global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R,
                                                              const uint32_t *offsets,
                                                              const uint32_t *sx,
                                                              const uint32_t *sy,
                                                              const uint32_t *sz,
                                                              float *regions) {
-    __shared__ int acc[2 * kMaxR * kMaxR * kMaxR];
-    const int R3 = R * R * R, ncell = ngrids * R3;
-    const int single = ngrids == 1;
+    __shared__ int acc[NG * kMaxR * kMaxR * kMaxR];
+    const int R3 = R * R * R, ncell = NG * R3;
+    constexpr int single = NG == 1;
     const int per = deposit_per_thread<ORDER>(single);
     const int64_t tile = blockIdx.x;
     float *out = regions + tile * (int64_t)ncell;
@@ -505,8 +531,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R, int n
         for (int i = threadIdx.x; i < ncell; i += blockDim.x) acc[i] = 0;
         const uint32_t rend = min(e, r0 + round);
         for (uint32_t c0 = r0; c0 < rend; c0 += chunk)
-            deposit_chunk<ORDER>(acc, R, R3, ngrids, single, per, fix, c0, min(rend, c0 + chunk),
-                                 sx, sy, sz);
+            deposit_chunk<ORDER, NG>(acc, R, R3, per, fix, c0, min(rend, c0 + chunk), sx, sy, sz);
         __syncthreads();
         RegionCursor rc;
         rc.init(threadIdx.x, R);
@@ -676,13 +701,14 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
         TimedScope ts(timer, "pbi4_deposit", s);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3((unsigned)p.ntiles), dim3(kDepositThreads), 0, s, p.T,
-                               p.R, p.ngrids, offsets, sx, sy, sz, regions);
+                               p.R, offsets, sx, sy, sz, regions);
         };
+        const bool two = p.ngrids == 2;
         switch (p.order) {
-            case 1: launch(k_deposit<1>); break;
-            case 2: launch(k_deposit<2>); break;
-            case 3: launch(k_deposit<3>); break;
-            default: launch(k_deposit<4>); break;
+            case 1: two ? launch(k_deposit<1, 2>) : launch(k_deposit<1, 1>); break;
+            case 2: two ? launch(k_deposit<2, 2>) : launch(k_deposit<2, 1>); break;
+            case 3: two ? launch(k_deposit<3, 2>) : launch(k_deposit<3, 1>); break;
+            default: two ? launch(k_deposit<4, 2>) : launch(k_deposit<4, 1>); break;
         }
     }
     {
