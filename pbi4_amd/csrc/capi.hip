@@ -354,7 +354,7 @@ BinPlan make_bin_plan(int N, int nbins_pk, int nshell, int multipoles, const dou
     for (int i = 0; i < 3; ++i) b.los[i] = (float)los[i];
     b.kF = (float)(double)(kTwoPiF32 / box);
     const int64_t rows = (int64_t)N * N;
-    b.nwg = (int)std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(rows, 16)));
+    b.nwg = (int)std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(rows, 16)));
     return b;
 }
 
@@ -448,6 +448,47 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
     launch_bk_finalize(d->multipoles ? 11 : 9, d->itot, tri, L.C, L.g.npair, L.g.ncol, d->nbins,
                        L.scale_exp, L.shell_sum, sh.nmodes_d, counts, kf3, powSN, gridD,
                        ratio * ratio * ratio, (double)d->kcenter, (double)d->kbin, bout, s);
+}
+
+// Cross bispectrum workspace: the maps of field 1 then field 2 on the maps grid (2 nb maps), the
+// GEMM run with nb pair rows from field 1 and 2 nb columns (field 2 in columns nb .. 2nb-1).
+struct BkCrossLayout {
+    int Np;
+    int64_t K, Kstride;
+    GemmPlan g;
+    float *bin_partials;
+    double *shell_sum;   // 2 nb: field 1, field 2
+    float *scale_exp;    // 2 nb
+    float2 *spectra;
+    float *maps;
+    void *fftwork;
+    float *gemm_partials;
+    double *C;
+};
+
+BkCrossLayout bk_cross_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
+    PBI4_CHECK(!d.multipoles, PBI4_EINVAL, "the cross bispectrum has no multipoles");
+    ShellInfo &sh = get_shells(ctx, d.grid, d.nbins, d.kbin, d.kcenter);
+    BkCrossLayout L{};
+    L.Np = d.maps_grid ? d.maps_grid : choose_maps_grid(d.grid, sh.R);
+    PBI4_CHECK(d.maps_grid == 0 || L.Np == d.grid || 3 * sh.R < L.Np, PBI4_EINVAL,
+               "maps_grid too small for the shells (needs > 3 * max|q| = " +
+                   std::to_string(3 * sh.R) + ")");
+    const int nmaps = 2 * d.nbins;
+    L.K = (int64_t)L.Np * L.Np * L.Np;
+    L.Kstride = (int64_t)align_up((size_t)L.K, 64);
+    L.g = make_gemm_plan(d.nbins, 2, L.K, L.Kstride, d.precision);
+    BinPlan b = make_bin_plan(d.grid, 0, d.nbins, 0, d.los, 1.0f);
+    FftPlan &p = get_plan(ctx, 1, L.Np, nmaps, L.Kstride);
+    L.bin_partials = a.take<float>(bin_partials_floats(b));
+    L.shell_sum = a.take<double>(nmaps);
+    L.scale_exp = a.take<float>(nmaps);
+    L.spectra = a.take<float2>((size_t)nmaps * L.Np * L.Np * (L.Np / 2 + 1));
+    L.maps = a.take<float>((size_t)nmaps * L.Kstride);
+    L.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
+    L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
+    L.C = a.take<double>((size_t)L.g.npair * L.g.ncol);
+    return L;
 }
 }  // namespace
 
@@ -732,6 +773,73 @@ int pbi4_measure_bk(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *counts, c
         launch_bin(b, static_cast<const float2 *>(delta), L.bin_partials, s);
         launch_bin_finalize(b, L.bin_partials, 1.0, nullptr, L.shell_sum, s);
         bk_from_shells(ctx, d, L, sh, tri, counts, delta, delta2, delta4, bout, s);
+    });
+}
+
+size_t pbi4_bk_cross_workspace(pbi4_ctx *ctx, const pbi4_bk_desc *d) {
+    size_t r = 0;
+    guarded(ctx, [&] {
+        PBI4_CHECK(d, PBI4_EINVAL, "NULL argument");
+        check_bk(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        Arena a(nullptr, 0);
+        bk_cross_layout(ctx, *d, a);
+        r = a.used + 256;
+    });
+    return r;
+}
+
+int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *counts,
+                          const void *delta1, const void *delta2, double *bout, void *ws,
+                          size_t ws_bytes, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(d && counts && delta1 && delta2 && bout && ws, PBI4_EINVAL, "NULL argument");
+        check_bk(*d);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipStream_t s = (hipStream_t)stream;
+        ShellInfo &sh = get_shells(ctx, d->grid, d->nbins, d->kbin, d->kcenter);
+        int32_t *tri = get_tri_table(ctx, d->nbins, d->kbin, d->kcenter, d->iopen, 1, d->itot);
+        Arena a(ws, ws_bytes);
+        BkCrossLayout L = bk_cross_layout(ctx, *d, a);
+        const int nb = d->nbins;
+        const size_t spec = (size_t)L.Np * L.Np * (L.Np / 2 + 1);
+        const void *fields[2] = {delta1, delta2};
+        for (int f = 0; f < 2; ++f) {
+            // shell powers (Parseval) -> map scales; shell filter into maps f*nb .. f*nb+nb-1
+            BinPlan b = make_bin_plan(d->grid, 0, nb, 0, d->los, 1.0f);
+            b.shell_table = sh.table_d;
+            b.max_d2 = sh.max_d2;
+            launch_bin(b, static_cast<const float2 *>(fields[f]), L.bin_partials, s);
+            launch_bin_finalize(b, L.bin_partials, 1.0, nullptr, L.shell_sum + f * nb, s);
+            FilterPlan fp{};
+            fp.N = d->grid;
+            fp.Np = L.Np;
+            fp.R = sh.R;
+            fp.nshell = nb;
+            fp.nell = 1;
+            for (int i = 0; i < 3; ++i) fp.los[i] = (float)d->los[i];
+            fp.shell_table = sh.table_d;
+            fp.max_d2 = sh.max_d2;
+            TimedScope ts(&ctx->timer, "pbi4_filter", s);
+            launch_filter(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,
+                          sh.nmodes_d, L.spectra + f * nb * spec, L.scale_exp + f * nb, s);
+        }
+        FftPlan &p = get_plan(ctx, 1, L.Np, 2 * nb, L.Kstride);
+        {
+            TimedScope ts(&ctx->timer, "pbi4_c2r", s);
+            fft_exec(p, L.spectra, L.maps, L.fftwork, s);
+        }
+        launch_gemm(L.g, L.maps, L.gemm_partials, s, &ctx->timer);
+        launch_gemm_reduce(L.g, L.gemm_partials, L.C, s);
+        // kf3 = kf**3 in REAL(4) (bispectrumI4.f90:569)
+        volatile float kfv = d->kf;
+        volatile float k2 = kfv * kfv;
+        volatile float k3 = k2 * kfv;
+        const double gridD = (double)d->grid * d->grid * d->grid;
+        const double ratio = (double)d->grid / (double)L.Np;
+        launch_bk_cross_finalize(d->itot, tri, L.C, L.g.ncol, nb, L.scale_exp, counts, (double)k3,
+                                 gridD, ratio * ratio * ratio, (double)d->kcenter, (double)d->kbin,
+                                 bout, s);
     });
 }
 

@@ -44,10 +44,34 @@ __global__ void k_combine_interlaced(int N, float theta, const float *wtab, floa
     const int icx = (N - ix) % N, icy = (N - iy) % N, icz = (N - iz) % N;
     const int mask = (ix == icx ? 1 : 0) | (iy == icy ? 2 : 0) | (iz == icz ? 4 : 0);
 
+    // The 8 values F(X, Y, Z), X in {ix, icx}, Y in {iy, icy}, Z in {iz, icz}, come from the 4
+    // half-spectrum entries (ix, Y, Z) of each transform: for X = icx > N/2,
+    // F(icx, Y, Z) = conj of the entry at the mirrored (Y, Z) (read_F).  Loaded unconditionally
+    // as one batch.
+    float2 g1[2][2], g2[2][2];
+#pragma unroll
+    for (int yb = 0; yb < 2; ++yb)
+#pragma unroll
+        for (int zb = 0; zb < 2; ++zb) {
+            const int64_t o = ((int64_t)(zb ? icz : iz) * N + (yb ? icy : iy)) * h + ix;
+            g1[yb][zb] = G1[o];
+            g2[yb][zb] = G2[o];
+        }
+    const bool mirror = icx > N / 2;
     float2 orig[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        orig[k] = read_F(N, h, G1, G2, (k & 1) ? icx : ix, (k & 2) ? icy : iy, (k & 4) ? icz : iz);
+    for (int k = 0; k < 8; ++k) {
+        const int xb = k & 1, yb = (k >> 1) & 1, zb = (k >> 2) & 1;
+        float2 a, b;
+        if (xb && mirror) {
+            a = conjf2(g1[1 - yb][1 - zb]);
+            b = conjf2(g2[1 - yb][1 - zb]);
+        } else {
+            a = g1[yb][zb];
+            b = g2[yb][zb];
+        }
+        orig[k] = make_float2(a.x - b.y, a.y + b.x);
+    }
 
     auto cm = [&](int s) {  // i * exp(-i theta s)
         float sn, cs;
@@ -143,16 +167,39 @@ __device__ __forceinline__ void seg_prefix_sum(int key, float (&v)[NV], int lane
     seg_step<0x143>(key, v, lane >= 32);
 }
 
-// Contributions of one mode (x, y, z) of the half spectrum: P(k) bin key with (count, k, P0, P2,
-// P4) and shell key with its weighted power.
+// Raw inputs of one mode, loaded unconditionally (indices clamped into range, absent optional
+// fields read delta instead) so a step's loads issue as one batch; a guarded load becomes a
+// branch with a wait after it.
+struct ModeLoads {
+    float2 d, e2, e4, ec;
+    int bin, shell;
+};
+
+__device__ __forceinline__ ModeLoads mode_loads(const BinPlan &b, const float2 *delta, int64_t row,
+                                                int h, int x, int aky, int akz) {
+    ModeLoads m;
+    const int64_t o = row * h + x;
+    m.d = delta[o];
+    m.e2 = (b.f2 ? b.f2 : delta)[o];
+    m.e4 = (b.f4 ? b.f4 : delta)[o];
+    m.ec = (b.cross ? b.cross : delta)[o];
+    m.bin = b.nbins_pk > 0 ? (int)b.pk_table[((int64_t)akz * h + aky) * h + x] : 0;
+    const int d2 = x * x + aky * aky + akz * akz;
+    m.shell = b.nshell > 0 ? (int)b.shell_table[min(d2, b.max_d2)] : 0;
+    if (d2 > b.max_d2) m.shell = 0;
+    return m;
+}
+
+// Contributions of one mode: P(k) bin key with (count, k, P0, P2, P4) and shell key with its
+// weighted power.
 struct ModeTerms {
     int key, skey;
     float v[5], sv[1];
 };
 
-__device__ __forceinline__ ModeTerms mode_terms(const BinPlan &b, const float2 *delta, int64_t row,
-                                                int N, int h, int x, int y, int ky, int aky,
-                                                int z, int kz, int akz, float lt, bool valid) {
+__device__ __forceinline__ ModeTerms mode_terms(const BinPlan &b, const ModeLoads &m, int N, int x,
+                                                int y, int ky, int aky, int z, int kz, int akz,
+                                                float lt, bool valid) {
     ModeTerms t;
     t.key = -1;
     t.skey = -1;
@@ -160,32 +207,29 @@ __device__ __forceinline__ ModeTerms mode_terms(const BinPlan &b, const float2 *
     for (int q = 0; q < 5; ++q) t.v[q] = 0.0f;
     t.sv[0] = 0.0f;
     if (!valid) return t;
-    const float2 d = delta[row * h + x];
+    const float2 d = m.d;
     const float p = d.x * d.x + d.y * d.y;
     const int d2 = x * x + aky * aky + akz * akz;
     if (b.nbins_pk > 0) {
         const bool sel = x != 0 || ky > 0 || (ky == 0 && kz > 0);
-        const int bin = sel ? (int)b.pk_table[((int64_t)akz * h + aky) * h + x] : 0;
+        const int bin = sel ? m.bin : 0;
         t.key = bin;
         if (bin > 0) {
-            auto dot = [&](const float2 *f) {   // Re(conj(d) * f) at this mode
-                const float2 e = f[row * h + x];
-                return d.x * e.x + d.y * e.y;
-            };
+            auto dot = [&](float2 e) { return d.x * e.x + d.y * e.y; };   // Re(conj(d) e)
             const float k2 = (float)d2;
             const float lk = b.los[0] * (float)x + lt;
             const float mu2 = k2 > 0.0f ? lk * lk / k2 : 0.0f;
             t.v[0] = 1.0f;
             t.v[1] = b.kF * sqrtf(k2);
-            t.v[2] = (!b.multipoles && b.cross) ? dot(b.cross) : p;
+            t.v[2] = (!b.multipoles && b.cross) ? dot(m.ec) : p;
             if (b.multipoles) {
-                t.v[3] = b.f2 ? dot(b.f2) : (-0.5f + 1.5f * mu2) * p;
-                t.v[4] = b.f4 ? dot(b.f4) : (0.375f + mu2 * (-3.75f + 4.375f * mu2)) * p;
+                t.v[3] = b.f2 ? dot(m.e2) : (-0.5f + 1.5f * mu2) * p;
+                t.v[4] = b.f4 ? dot(m.e4) : (0.375f + mu2 * (-3.75f + 4.375f * mu2)) * p;
             }
         }
     }
     if (b.nshell > 0) {
-        const int s = d2 <= b.max_d2 ? (int)b.shell_table[d2] : 0;
+        const int s = m.shell;
         t.skey = s;
         if (s > 0) {
             const bool planex = x == 0 || x == N / 2;
@@ -219,10 +263,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
         const float lt = b.los[1] * (float)ky + b.los[2] * (float)kz;
         for (int x0 = 0; x0 < half; x0 += 64) {
             const int x = x0 + lane;
-            ModeTerms t = mode_terms(b, delta, row, N, h, x, y, ky, aky, z, kz, akz, lt, x < half);
+            const ModeLoads ml = mode_loads(b, delta, row, h, min(x, half - 1), aky, akz);
+            const ModeLoads mn = mode_loads(b, delta, row, h, half, aky, akz);   // Nyquist mode
+            ModeTerms t = mode_terms(b, ml, N, x, y, ky, aky, z, kz, akz, lt, x < half);
             const bool nyq_lane = x == half - 1;
             ModeTerms tn;
-            if (nyq_lane) tn = mode_terms(b, delta, row, N, h, half, y, ky, aky, z, kz, akz, lt, true);
+            if (nyq_lane) tn = mode_terms(b, mn, N, half, y, ky, aky, z, kz, akz, lt, true);
             const int last = min(63, half - 1 - x0);      // highest valid lane of this step
             if (b.nbins_pk > 0) {
                 seg_prefix_sum<5>(t.key, t.v, lane);
@@ -254,23 +300,44 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
     }
 }
 
-// FP64 combine of the per-work-group partials: 32 bins per block, the work-group range split
-// over 8 threads in fixed sub-ranges and summed in a fixed order; then pout / shell sums.
-__global__ __launch_bounds__(256) void k_bin_finalize(BinPlan b, const float *partials,
-                                                      double kF3, double *pk_out,
-                                                      double *shell_sum) {
-    __shared__ double red[8][32][5];
+// FP64 combine of the per-work-group partials: kFinBins bins per block, the work-group range
+// split over kFinParts threads in fixed sub-ranges (loads batched kFinBatch at a time, summed in
+// work-group order) and the parts summed in a fixed order; then pout / shell sums.
+constexpr int kFinBins = 8, kFinParts = 32, kFinBatch = 8;
+
+__global__ __launch_bounds__(kFinBins * kFinParts) void k_bin_finalize(BinPlan b,
+                                                                       const float *partials,
+                                                                       double kF3, double *pk_out,
+                                                                       double *shell_sum) {
+    __shared__ double red[kFinParts][kFinBins][5];
     const int nq = 5 * b.nbins_pk + b.nshell;
-    const int lo = threadIdx.x & 31, part = threadIdx.x >> 5;
-    const int i = blockIdx.x * 32 + lo;
-    const int per = (b.nwg + 7) / 8, g0 = part * per, g1 = min(b.nwg, g0 + per);
+    const int lo = threadIdx.x % kFinBins, part = threadIdx.x / kFinBins;
+    const int i = blockIdx.x * kFinBins + lo;
+    const int per = (b.nwg + kFinParts - 1) / kFinParts;
+    const int g0 = part * per, g1 = min(b.nwg, g0 + per);
+    const bool pk = i < b.nbins_pk, sh = !pk && i < b.nbins_pk + b.nshell;
+    const int nv = pk ? 5 : 1;
+    // quantity q of bin i: column q * nbins_pk + i (P(k) bins) or 4 nbins_pk + i (shells)
+    const int col0 = pk ? i : 4 * b.nbins_pk + i;
+    const int cstride = pk ? b.nbins_pk : 0;
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (i < b.nbins_pk) {
-        for (int g = g0; g < g1; ++g)
+    if (pk || sh) {
+        for (int g = g0; g < g1; g += kFinBatch) {
+            float v[kFinBatch][5];
 #pragma unroll
-            for (int q = 0; q < 5; ++q) s[q] += (double)partials[(int64_t)g * nq + q * b.nbins_pk + i];
-    } else if (i < b.nbins_pk + b.nshell) {
-        for (int g = g0; g < g1; ++g) s[0] += (double)partials[(int64_t)g * nq + 4 * b.nbins_pk + i];
+            for (int k = 0; k < kFinBatch; ++k) {
+                const int gg = min(g + k, g1 - 1);
+#pragma unroll
+                for (int q = 0; q < 5; ++q)
+                    v[k][q] = q < nv ? partials[(int64_t)gg * nq + col0 + q * cstride] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < kFinBatch; ++k)
+                if (g + k < g1) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) s[q] += (double)v[k][q];
+                }
+        }
     }
 #pragma unroll
     for (int q = 0; q < 5; ++q) red[part][lo][q] = s[q];
@@ -279,10 +346,10 @@ __global__ __launch_bounds__(256) void k_bin_finalize(BinPlan b, const float *pa
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         double r = red[0][lo][q];
-        for (int w = 1; w < 8; ++w) r += red[w][lo][q];
+        for (int w = 1; w < kFinParts; ++w) r += red[w][lo][q];
         s[q] = r;
     }
-    if (i < b.nbins_pk) {
+    if (pk) {
         double *p = pk_out + 5 * (int64_t)i;
         const double co = s[0];
         p[0] = s[1] / co;
@@ -296,7 +363,7 @@ __global__ __launch_bounds__(256) void k_bin_finalize(BinPlan b, const float *pa
             p[3] = 0.0;
             p[4] = 0.0;
         }
-    } else if (i < b.nbins_pk + b.nshell) {
+    } else if (sh) {
         shell_sum[i - b.nbins_pk] = s[0];
     }
 }
@@ -520,8 +587,8 @@ void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStrea
 void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, double *pk_out,
                          double *shell_sum, hipStream_t s) {
     const int n = b.nbins_pk + b.nshell;
-    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, 32)), dim3(256), 0, s, b, partials,
-                       kF3, pk_out, shell_sum);
+    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, kFinBins)),
+                       dim3(kFinBins * kFinParts), 0, s, b, partials, kF3, pk_out, shell_sum);
     PBI4_HIP(hipGetLastError());
 }
 
