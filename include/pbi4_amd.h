@@ -167,6 +167,22 @@ int pbi4_measure_bk(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *counts, c
                     const void *delta2, const void *delta4, double *bout, void *ws,
                     size_t ws_bytes, void *stream);
 
+/* ------------------------------------------------------------------ cross bispectrum
+ * Replaces bispectrumI4.measure_bk_cross (bispectrumI4.f90:518-761), called from
+ * MEASUREMENTS.bispectrum when params['cross'] is set (pbi4/pbi4.py:481-496).
+ * B(k_l, k_j, k_i) = kf^3 sum_x B_i(x) A_j(x) A_l(x) / N_tr with A the shell maps of delta1 and
+ * B those of delta2, over the cross loop's triangles (i up to min(nbins, j + l - 1 + ishift),
+ * :733-735).  Same shells, maps grid, precision and GEMM as pbi4_measure_bk, which runs with
+ * the nb field-1 maps as pair rows and the 2 nb maps (field 1, then field 2) as columns.
+ * d->multipoles must be 0; d->itot is the cross loop's count; counts from
+ * pbi4_count_triangles with cross = 1.  bout: device double, Fortran order (5, itot):
+ * k_l, k_j, k_i, B, N_tr / N^3 (REAL(4)-rounded), no shot noise (:751-755).
+ */
+size_t pbi4_bk_cross_workspace(pbi4_ctx *ctx, const pbi4_bk_desc *d);
+int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *counts,
+                          const void *delta1, const void *delta2, double *bout, void *ws,
+                          size_t ws_bytes, void *stream);
+
 /* ------------------------------------------------------------------ fused P_l + B_l
  * pbi4_measure_pk followed by pbi4_measure_bk on the same delta, with one binning pass that
  * produces the P_l bins and the shell powers together (the serving path).  Same outputs as

@@ -529,6 +529,32 @@ def measure_bk(nbins, grid, kbin, kcenter, iopen, kf, itot, npart, counts, dcl,
     return bout
 
 
+def measure_bk_cross(nbins, grid, kbin, kcenter, iopen, kf, itot, npart, counts, dcl1, dcl2,
+                     workers=None):
+    """measure_bk_cross (bispectrumI4.f90:518-761): rows k_l, k_j, k_i, kf3 sum_x B_i A_j A_l / N_tr
+    and N_tr / N^3 (:751-755), with A the shell maps of dcl1 (:644-657), B those of dcl2
+    (:691-703), over the cross loop's wider range of i (:733-735).  No shot noise and no power
+    columns; npart is unused, as in the reference."""
+    counts = np.asarray(counts, dtype=np.float32).astype(np.float64)   # f2py REAL(4) coercion
+    tl = triangle_list(nbins, kbin, kcenter, iopen, True)
+    assert len(tl) == itot
+    kb = np.float64(np.float32(kbin))
+    kc = np.float64(np.float32(kcenter))
+    tk = kc + np.arange(nbins, dtype=np.float64) * kb
+    kf3 = kf3_bk(kf)
+    gridD = np.float64(grid) ** 3
+    A, _ = shell_maps(dcl1, grid, nbins, kbin, kcenter, workers)
+    B, _ = shell_maps(dcl2, grid, nbins, kbin, kcenter, workers)
+    l, j, i = tl[:, 0] - 1, tl[:, 1] - 1, tl[:, 2] - 1
+    idx = np.ascontiguousarray(np.stack([j, l, i], axis=1), dtype=np.int32)
+    s = triple_sums(A, B, idx, workers)
+    bout = np.zeros((5, itot))
+    bout[0], bout[1], bout[2] = tk[l], tk[j], tk[i]
+    bout[3] = kf3 * s / counts
+    bout[4] = (counts / gridD).astype(np.float32).astype(np.float64)
+    return bout
+
+
 def count_triangles(nbins, grid, kbin, kcenter, iopen, cross=False, workers=None):
     """FFT form of count_triangles (bispectrumI4.f90:808-908), FP64."""
     idist = shell_index_grid(grid, kbin, kcenter)

@@ -72,6 +72,9 @@ SIGNATURES = {
     "pbi4_bk_maps_grid": (c_int, [c_void_p, P(BkDesc)]),
     "pbi4_measure_bk": (c_int, [c_void_p, P(BkDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pbi4_bk_cross_workspace": (c_size_t, [c_void_p, P(BkDesc)]),
+    "pbi4_measure_bk_cross": (c_int, [c_void_p, P(BkDesc), c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_size_t, c_void_p]),
     "pbi4_pk_bk_workspace": (c_size_t, [c_void_p, P(PkDesc), P(BkDesc)]),
     "pbi4_measure_pk_bk": (c_int, [c_void_p, P(PkDesc), P(BkDesc), c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_size_t, c_void_p]),

@@ -52,5 +52,14 @@ def measure_bk_multipoles(nbins, grid, kbin, kcenter, iopen, kf, npool, itot, np
 
 
 def measure_bk_cross(nbins, grid, kbin, kcenter, iopen, kf, npool, itot, npart, counts, dcl1, dcl2):
-    """bispectrumI4.f90:518-761 (cross bispectrum): not implemented."""
-    raise Pbi4Error("measure_bk_cross is not implemented")
+    """bispectrumI4.f90:518-761 -> bout (5, itot): k1, k2, k3, B (dcl2 on the third bin), N_tr."""
+    ctx = default_context()
+    d = ctx.device
+    desc = ctx.bk_desc(int(nbins), int(grid), float(kbin), float(kcenter), bool(iopen),
+                       np.float32(kf), int(itot), int(npart), False, (0.0, 0.0, 1.0), PRECISION,
+                       MAPS_GRID)
+    c = torch.from_numpy(np.ascontiguousarray(np.asarray(counts).astype(np.float32))).to(d)
+    if c.numel() != int(itot):
+        raise Pbi4Error("counts must have itot entries")
+    return np.asfortranarray(ctx.bk_cross(half_from_array(dcl1, d), half_from_array(dcl2, d), c,
+                                          desc).cpu().numpy().T)

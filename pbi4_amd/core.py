@@ -187,6 +187,29 @@ class Context:
                                           ws.numel(), _stream(stream)))
         return out
 
+    def bk_cross_workspace(self, desc):
+        n = lib().pbi4_bk_cross_workspace(self.h, ctypes.byref(desc))
+        if n == 0:
+            raise Pbi4Error("pbi4_bk_cross_workspace: " + lib().pbi4_last_error(self.h).decode())
+        return n
+
+    def bk_cross(self, delta1, delta2, counts, desc, out=None, ws=None, stream=None):
+        """Cross bispectrum (bispectrumI4.f90:518-761) as an (itot, 5) float64 tensor: k_l, k_j,
+        k_i, B, N_tr/N^3.  counts: the cross loop's REAL(4) counts as a float32 device tensor."""
+        _check_dev(delta1, "delta1", torch.complex64)
+        _check_dev(delta2, "delta2", torch.complex64)
+        _check_dev(counts, "counts", torch.float32)
+        if counts.numel() != desc.itot:
+            raise Pbi4Error("counts must have itot entries")
+        if out is None:
+            out = torch.empty((desc.itot, 5), dtype=torch.float64, device=self.device)
+        if ws is None:
+            ws = self._ws(self.bk_cross_workspace(desc))
+        self._check(lib().pbi4_measure_bk_cross(self.h, ctypes.byref(desc), _ptr(counts),
+                                                _ptr(delta1), _ptr(delta2), _ptr(out), _ptr(ws),
+                                                ws.numel(), _stream(stream)))
+        return out
+
     # ------------------------------------------------------------- fused P_l + B_l
     def pk_bk_workspace(self, pk_desc, bk_desc):
         n = lib().pbi4_pk_bk_workspace(self.h, ctypes.byref(pk_desc), ctypes.byref(bk_desc))

@@ -131,6 +131,11 @@ void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double
                         double gridD, double sum_scale, double kcenter, double kbin, double *bout,
                         hipStream_t s);
 
+void launch_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, int ncol, int nb,
+                              const float *scale_exp, const float *counts, double kf3,
+                              double gridD, double sum_scale, double kcenter, double kbin,
+                              double *bout, hipStream_t s);
+
 void count_pairs_gpu(int N, int nb, const int32_t *modes_d, int64_t nmodes, const int64_t *shell_off,
                      const int32_t *canon_d, const int32_t *canon_w, int64_t ncanon,
                      const int16_t *shell_table_d, int max_d2, unsigned long long *T_d,

@@ -300,55 +300,41 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
     }
 }
 
-// FP64 combine of the per-work-group partials: kFinBins bins per block, the work-group range
-// split over kFinParts threads in fixed sub-ranges (loads batched kFinBatch at a time, summed in
-// work-group order) and the parts summed in a fixed order; then pout / shell sums.
-constexpr int kFinBins = 8, kFinParts = 32, kFinBatch = 8;
+// FP64 combine of the per-work-group partials: one block per bin; thread t sums work-groups
+// t, t + 256, ... in that order, then a fixed pairwise tree over the threads (deterministic);
+// then pout / shell sums.
+constexpr int kFinThreads = 256;
 
-__global__ __launch_bounds__(kFinBins * kFinParts) void k_bin_finalize(BinPlan b,
-                                                                       const float *partials,
-                                                                       double kF3, double *pk_out,
-                                                                       double *shell_sum) {
-    __shared__ double red[kFinParts][kFinBins][5];
+__global__ __launch_bounds__(kFinThreads) void k_bin_finalize(BinPlan b, const float *partials,
+                                                              double kF3, double *pk_out,
+                                                              double *shell_sum) {
+    __shared__ double red[5][kFinThreads];
     const int nq = 5 * b.nbins_pk + b.nshell;
-    const int lo = threadIdx.x % kFinBins, part = threadIdx.x / kFinBins;
-    const int i = blockIdx.x * kFinBins + lo;
-    const int per = (b.nwg + kFinParts - 1) / kFinParts;
-    const int g0 = part * per, g1 = min(b.nwg, g0 + per);
-    const bool pk = i < b.nbins_pk, sh = !pk && i < b.nbins_pk + b.nshell;
+    const int i = blockIdx.x;
+    const bool pk = i < b.nbins_pk;
     const int nv = pk ? 5 : 1;
     // quantity q of bin i: column q * nbins_pk + i (P(k) bins) or 4 nbins_pk + i (shells)
     const int col0 = pk ? i : 4 * b.nbins_pk + i;
     const int cstride = pk ? b.nbins_pk : 0;
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (pk || sh) {
-        for (int g = g0; g < g1; g += kFinBatch) {
-            float v[kFinBatch][5];
+    for (int g = threadIdx.x; g < b.nwg; g += kFinThreads) {
 #pragma unroll
-            for (int k = 0; k < kFinBatch; ++k) {
-                const int gg = min(g + k, g1 - 1);
-#pragma unroll
-                for (int q = 0; q < 5; ++q)
-                    v[k][q] = q < nv ? partials[(int64_t)gg * nq + col0 + q * cstride] : 0.0f;
-            }
-#pragma unroll
-            for (int k = 0; k < kFinBatch; ++k)
-                if (g + k < g1) {
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) s[q] += (double)v[k][q];
-                }
-        }
+        for (int q = 0; q < 5; ++q)
+            if (q < nv) s[q] += (double)partials[(int64_t)g * nq + col0 + q * cstride];
     }
 #pragma unroll
-    for (int q = 0; q < 5; ++q) red[part][lo][q] = s[q];
+    for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = s[q];
     __syncthreads();
-    if (part != 0) return;
+    for (int w = kFinThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        double r = red[0][lo][q];
-        for (int w = 1; w < kFinParts; ++w) r += red[w][lo][q];
-        s[q] = r;
+            for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        }
+        __syncthreads();
     }
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) s[q] = red[q][0];
     if (pk) {
         double *p = pk_out + 5 * (int64_t)i;
         const double co = s[0];
@@ -363,7 +349,7 @@ __global__ __launch_bounds__(kFinBins * kFinParts) void k_bin_finalize(BinPlan b
             p[3] = 0.0;
             p[4] = 0.0;
         }
-    } else if (sh) {
+    } else {
         shell_sum[i - b.nbins_pk] = s[0];
     }
 }
@@ -587,8 +573,9 @@ void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStrea
 void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, double *pk_out,
                          double *shell_sum, hipStream_t s) {
     const int n = b.nbins_pk + b.nshell;
-    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, kFinBins)),
-                       dim3(kFinBins * kFinParts), 0, s, b, partials, kF3, pk_out, shell_sum);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)n), dim3(kFinThreads), 0, s, b, partials, kF3,
+                       pk_out, shell_sum);
     PBI4_HIP(hipGetLastError());
 }
 

@@ -24,7 +24,6 @@ constexpr int kPadB = 8;              // bf16 padding per LDS row
 constexpr int kMaxCF = 12;            // column fragments (ncol <= 192)
 constexpr int kXcds = 8;              // MI355X: work-groups are dealt round-robin to 8 XCDs
 
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void pair_of(int p, int &i, int &j) {
     // p = j(j+1)/2 + i with 0 <= i <= j
@@ -34,6 +33,8 @@ __device__ __forceinline__ void pair_of(int p, int &i, int &j) {
     j = jj;
     i = p - jj * (jj + 1) / 2;
 }
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 template <int CF>
 struct GemmTile {
@@ -281,6 +282,29 @@ __global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const 
     }
 }
 
+// Cross bispectrum (bispectrumI4.f90:733-755): triangle m = (l, j, i) takes the pair (j <= l)
+// of field-1 maps (GEMM rows) and field-2 map i (GEMM column nb + i - 1); rows k_l, k_j, k_i,
+// kf3 sum / N_tr, N_tr / N^3.  scale_exp holds the field-1 scales, then the field-2 scales.
+__global__ void k_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, int ncol,
+                                    int nb, const float *scale_exp, const float *counts,
+                                    double kf3, double gridD, double sum_scale, double kcenter,
+                                    double kbin, double *bout) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= itot) return;
+    const int l = tri[3 * m], j = tri[3 * m + 1], i = tri[3 * m + 2];   // 1-based
+    const int64_t p = (int64_t)(l - 1) * l / 2 + (j - 1);
+    const double unscale =
+        sum_scale * ldexp(1.0, -(int)(scale_exp[j - 1] + scale_exp[l - 1] + scale_exp[nb + i - 1]));
+    const double cnt = (double)counts[m];
+    const double sum = C[p * ncol + nb + (i - 1)] * unscale;
+    double *b = bout + 5 * m;
+    b[0] = kcenter + (double)(l - 1) * kbin;
+    b[1] = kcenter + (double)(j - 1) * kbin;
+    b[2] = kcenter + (double)(i - 1) * kbin;
+    b[3] = kf3 * sum / cnt;
+    b[4] = (double)(float)(cnt / gridD);
+}
+
 // ------------------------------------------------------------------ exact triangle counts
 // One work-group per canonical q1 (|q1x| >= |q1y| >= |q1z| >= 0 up to the lattice symmetry
 // group, weighted by its orbit size): for every q2 in shells j >= shell(q1), the shell l of
@@ -426,6 +450,17 @@ void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double
     hipLaunchKernelGGL(k_bk_finalize, dim3((unsigned)ceil_div(itot, 128)), dim3(128), 0, s, rows,
                        itot, tri, C, ncol, nb, scale_exp, shell_sum, nmodes, counts, kf3, powSN,
                        gridD, sum_scale, kcenter, kbin, bout);
+    PBI4_HIP(hipGetLastError());
+}
+
+void launch_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, int ncol, int nb,
+                              const float *scale_exp, const float *counts, double kf3,
+                              double gridD, double sum_scale, double kcenter, double kbin,
+                              double *bout, hipStream_t s) {
+    if (itot <= 0) return;
+    hipLaunchKernelGGL(k_bk_cross_finalize, dim3((unsigned)ceil_div(itot, 256)), dim3(256), 0, s,
+                       itot, tri, C, ncol, nb, scale_exp, counts, kf3, gridD, sum_scale, kcenter,
+                       kbin, bout);
     PBI4_HIP(hipGetLastError());
 }
 

@@ -329,10 +329,26 @@ class MEASUREMENTS:
         if params is None:
             params = self.params
         ctx = _ctx(params)
-        if params.get("cross"):
-            raise Pbi4Error("the cross bispectrum (measure_bk_cross) is not implemented")
         N = int(params["grid"])
         d = _as_density(dcl, params)
+        if params.get("cross"):
+            # pbi4/pbi4.py:481-496: dcl2 is the second field; B uses it on the third bin
+            if dcl2 is None:
+                raise Pbi4Error("the cross bispectrum needs the second field as dcl2")
+            d2 = _as_density(dcl2, params)
+            desc = ctx.bk_desc(int(params["nbins_bisp"]), N, float(params["kbin"]),
+                               float(params["kcenter"]), bool(params["iopen"]),
+                               np.float32(params["kf"]), int(params["itot"]), int(npart), False,
+                               tuple(params["los"]), _precision(params),
+                               int(params.get("maps_grid", 0)))
+            c = torch.from_numpy(np.ascontiguousarray(np.asarray(counts).astype(np.float32))).to(ctx.device)
+            print("Computing cross bispectra")
+            bout = ctx.bk_cross(d.delta, d2.delta, c, desc).cpu().numpy().T
+            output = np.zeros(bout.shape[1], dtype=[(n, np.float32) for n in ("k1", "k2", "k3", "B0", "Ntr")])
+            for r, n in enumerate(("k1", "k2", "k3", "B0", "Ntr")):
+                output[n] = bout[r]
+            print("Done.\n")
+            return output
         mult = bool(params["multipoles"])
         desc = ctx.bk_desc(int(params["nbins_bisp"]), N, float(params["kbin"]),
                            float(params["kcenter"]), bool(params["iopen"]), np.float32(params["kf"]),
@@ -379,7 +395,11 @@ class MEASUREMENTS:
             params = self.params
         print("Saving bispectrum...\n")
         print("FILE: %s" % params["bs_outfile"])
-        if params["multipoles"] == 0:
+        if params.get("cross"):
+            header = "k1/kF k2/kF k3/kF      B0          N_tr\n"
+            cols = ["k1", "k2", "k3", "B0", "Ntr"]
+            fmt = ("%d", "%d", "%d", "%0.8e", "%0.8e")
+        elif params["multipoles"] == 0:
             header = "k1/kF k2/kF k3/kF P(k1) P(k2)       P(k3)       B0+BSN        BSN    N_tr\n"
             cols = ["k1", "k2", "k3", "Pk1", "Pk2", "Pk3", "B0+BSN", "BSN", "Ntr"]
             fmt = ("%d", "%d", "%d") + ("%0.8e",) * 6

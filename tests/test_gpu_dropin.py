@@ -31,6 +31,16 @@ def _clustered(n, seed, box=1000.0):
     return p.astype(np.float32), v.astype(np.float32)
 
 
+def _second_tracer(pos, seed, jitter=0.004):
+    """A second catalogue of the same clustering: a random 3/4 of the particles, each moved by
+    a small random offset (in box units), as two tracers of one field.  Positions (3, n) in
+    [0, 1)."""
+    rng = np.random.default_rng(seed)
+    keep = rng.random(pos.shape[1]) < 0.75
+    q = pos[:, keep] + jitter * rng.standard_normal((3, int(keep.sum())))
+    return (q % 1.0).astype(pos.dtype)
+
+
 def _pk_close(mine, ref, rows):
     ok = ref[rows[-1]] > 0
     for r in rows[1:-1]:
@@ -117,6 +127,41 @@ def test_script_sequence(tmp_path, multipoles, precision):
     assert np.array_equal(ms.triangle_counts(), counts)
 
 
+def test_cross_bispectrum_measurements(tmp_path):
+    # MEASUREMENTS with params["cross"]: counts over the cross loop, B of two catalogues
+    # (pbi4/pbi4.py:472-496) and the five-column bs file (:579-582)
+    import pbi4_amd as pb
+    N, n, nb = 32, 40000, 6
+    pa = (_clustered(n, 12)[0].T % 1000.0 / 1000.0).astype(np.float32)
+    pb_ = _second_tracer(pa, 13)
+    nb_part = pb_.shape[1]
+    itc = O.itot_count(nb, 1.0, 1.0, True, True)
+    params = {"grid": N, "interpolation": 4, "interlacing": True, "box": 1000.0,
+              "kf": 2 * np.pi / 1000, "kN": np.pi / 1000 * N, "multipoles": False,
+              "los": np.array([0.0, 0.0, 1.0]), "cross": True, "nbins_bisp": nb, "kbin": 1.0,
+              "kcenter": 1.0, "iopen": True, "itot": itc, "npool": 4,
+              "cfile": str(tmp_path / "cfile.npz"), "bs_outfile": str(tmp_path / "bs.dat")}
+    den = pb.DENSITIES(params)
+    _, ga = den.assign_grid(pa)
+    _, gb = den.assign_grid(pb_)
+    da = den.compute_density(n, ga)
+    db = den.compute_density(nb_part, gb)
+    ms = pb.MEASUREMENTS(params)
+    counts = ms.triangle_counts()
+    bs = ms.bispectrum(n, counts, da, db)
+    assert bs.dtype.names == ("k1", "k2", "k3", "B0", "Ntr") and len(bs) == itc
+    ms.save_bispectrum(bs)
+    cols = np.loadtxt(params["bs_outfile"])
+    assert cols.shape == (itc, 5)
+    oa = O.compute_density(n, O.assign_double_grid(N, 4, n, pa.astype(np.float64)), N, 4, True, 1000.0)
+    ob = O.compute_density(nb_part, O.assign_double_grid(N, 4, nb_part, pb_.astype(np.float64)), N, 4,
+                           True, 1000.0)
+    ref = O.measure_bk_cross(nb, N, 1.0, 1.0, True, np.float32(params["kf"]), itc, n, counts, oa, ob)
+    mine = np.array([bs[c] for c in ("k1", "k2", "k3", "B0", "Ntr")], dtype=np.float64)
+    _bk_cross_close(mine, ref.astype(np.float32).astype(np.float64), oa, ob, N, nb,
+                    np.float32(params["kf"]), rtol=2e-4)
+
+
 def test_cross_power_spectrum():
     import pbi4_amd.powerI4 as P
     N = 32
@@ -196,6 +241,31 @@ def test_f2py_bispectrum_functions():
     d2, d4 = O.compute_density_multipoles(d, N, 1000.0, [0.0, 1.0, 0.0])
     _bk_close(B.measure_bk_multipoles(nb, N, 1.0, 1.0, True, kf, 4, itot, n, counts, d, d2, d4),
               O.measure_bk(nb, N, 1.0, 1.0, True, kf, itot, n, counts, d, d2, d4))
+    # cross bispectrum of two catalogues (bispectrumI4.f90:518-761)
+    pos_b = _second_tracer(pos, 10)
+    db = O.compute_density(pos_b.shape[1], O.assign_double_grid(N, 4, pos_b.shape[1], pos_b), N, 4,
+                           True, 1000.0)
+    itc = O.itot_count(nb, 1.0, 1.0, True, True)
+    cc = B.count_triangles(nb, N, 1.0, 1.0, True, True, 4, itc)
+    assert np.array_equal(cc, O.count_triangles_exact(nb, N, 1.0, 1.0, True, cross=True) * float(N) ** 3)
+    _bk_cross_close(B.measure_bk_cross(nb, N, 1.0, 1.0, True, kf, 4, itc, n, cc, d, db),
+                    O.measure_bk_cross(nb, N, 1.0, 1.0, True, kf, itc, n, cc, d, db), d, db, N, nb, kf)
+
+
+def _bk_cross_close(mine, ref, da, db, N, nb, kf, rtol=1e-4):
+    """k's and N_tr exact; B relative to max(|B|, sqrt(|P_a(j) P_a(l) P_b(i)| / N_tr))."""
+    assert mine.shape == ref.shape == (5, ref.shape[1])
+    for r in (0, 1, 2, 4):
+        assert np.array_equal(mine[r], ref[r]), r
+    kf3 = O.kf3_bk(kf)
+    pw = []
+    for dcl in (da, db):
+        maps, nmodes = O.shell_maps(dcl, N, nb, 1.0, 1.0)
+        pw.append(kf3 * np.sum(maps * maps, axis=1) / float(N) ** 3 / nmodes)
+    l, j, i = (ref[0] - 1).astype(int), (ref[1] - 1).astype(int), (ref[2] - 1).astype(int)
+    gauss = np.sqrt(np.abs(pw[0][j] * pw[0][l] * pw[1][i]) / np.maximum(ref[4], 1))
+    err = np.abs(mine[3] - ref[3]) / np.maximum(np.abs(ref[3]), gauss)
+    assert err.max() <= rtol, (err.max(), np.argmax(err))
 
 
 # ------------------------------------------------------------------ reference-layout views

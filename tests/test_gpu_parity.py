@@ -233,3 +233,41 @@ def test_bispectrum(ctx, prec, multipoles, maps_grid):
     else:
         ref = O.measure_bk(nb, N, kb, kc, True, kf, itot, n, counts, dcl)
     _bk_close(bout, ref, 11 if multipoles else 9)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("maps_grid", [0, 32])
+def test_bispectrum_cross(ctx, prec, maps_grid):
+    # two different clustered fields; B = kf^3 sum B_i A_j A_l / N_tr over the cross loop
+    N, n, box, nb, kb, kc = 32, 50000, 1000.0, 6, 1.0, 1.0
+    # two tracers of one clustered field: 3/4 of the particles, each moved by a small offset
+    pa = _pos(n, 77)
+    rng = np.random.default_rng(78)
+    keep = rng.random(n) < 0.75
+    pb = np.minimum(((pa[:, keep] + 0.004 * rng.standard_normal((3, int(keep.sum())))) % 1.0)
+                    .astype(np.float32), np.float32(1.0 - 2 ** -24))
+    nb_part = pb.shape[1]
+    da = ctx.density(ctx.assign(_dev(pa), N, 3, True), N, 3, True, box, n)
+    db = ctx.density(ctx.assign(_dev(pb), N, 3, True), N, 3, True, box, nb_part)
+    itot = O.itot_count(nb, kb, kc, True, True)
+    ntr = ctx.count_triangles(nb, N, kb, kc, True, True, itot)
+    assert np.array_equal(ntr, O.count_triangles_exact(nb, N, kb, kc, True, cross=True))
+    counts = (ntr.astype(np.float64) * N ** 3).astype(np.float32)
+    kf = np.float32(2 * np.pi / box)
+    desc = ctx.bk_desc(nb, N, kb, kc, True, kf, itot, n, False, (0.0, 0.0, 1.0), prec, maps_grid)
+    bout = ctx.bk_cross(da, db, _dev(counts), desc).cpu().numpy().T
+    _, dca = _oracle_density(pa, N, 3, True, box)
+    _, dcb = _oracle_density(pb, N, 3, True, box)   # npart taken from pb
+    ref = O.measure_bk_cross(nb, N, kb, kc, True, kf, itot, n, counts, dca, dcb)
+    for r in (0, 1, 2, 4):
+        assert np.array_equal(bout[r], ref[r]), r
+    kf3 = O.kf3_bk(kf)
+    pw = []
+    for dcl in (dca, dcb):
+        maps, nmodes = O.shell_maps(dcl, N, nb, kb, kc)
+        pw.append(kf3 * np.sum(maps * maps, axis=1) / float(N) ** 3 / nmodes)
+    l, j, i = (ref[0] - kc).astype(int), (ref[1] - kc).astype(int), (ref[2] - kc).astype(int)
+    gauss = np.sqrt(np.abs(pw[0][j] * pw[0][l] * pw[1][i]) / np.maximum(ref[4], 1))
+    scale = np.maximum(np.abs(ref[3]), gauss)
+    err = np.abs(bout[3] - ref[3]) / scale
+    assert err.max() <= 1e-4, (err.max(), np.argmax(err))

@@ -361,6 +361,61 @@ def test_bispectrum_three_plane_waves_closed_form():
     del sh
 
 
+def test_bk_cross_of_one_field_equals_auto_raw():
+    # sum_x M_i M_j M_l is symmetric: the cross estimate of a field with itself equals the auto
+    # estimate without shot noise (B0 + BSN - BSN) of the sorted triangle, and every triangle of
+    # the cross loop sorts into the auto loop; the exact counts agree the same way
+    N, npart, nb = 16, 5000, 5
+    pos = _rand_pos(npart, 21, clustered=True)
+    dcl = O.compute_density(npart, O.assign_double_grid(N, 3, npart, pos), N, 3, 1, 1000.0)
+    kf = np.float32(2 * np.pi / 1000.0)
+    ta = O.triangle_list(nb, 1.0, 1.0, True)
+    tc = O.triangle_list(nb, 1.0, 1.0, True, cross=True)
+    assert len(tc) == O.itot_count(nb, 1.0, 1.0, True, True) > len(ta)
+    ca = O.count_triangles_exact(nb, N, 1.0, 1.0, True).astype(np.float64) * N ** 3
+    cc = O.count_triangles_exact(nb, N, 1.0, 1.0, True, cross=True).astype(np.float64) * N ** 3
+    auto = O.measure_bk(nb, N, 1.0, 1.0, True, kf, len(ta), npart, ca, dcl)
+    cross = O.measure_bk_cross(nb, N, 1.0, 1.0, True, kf, len(tc), npart, cc, dcl, dcl)
+    where = {tuple(t): m for m, t in enumerate(ta)}
+    for m, (l, j, i) in enumerate(tc):
+        k = where[tuple(sorted((l, j, i), reverse=True))]
+        assert cc[m] == ca[k]
+        assert np.isclose(cross[3, m], auto[6, k] - auto[7, k], rtol=1e-10, atol=1e-12 * abs(auto[7, k]))
+        assert (cross[0, m], cross[1, m], cross[2, m]) == (float(l), float(j), float(i))
+
+
+def test_bk_cross_plane_waves_closed_form():
+    # field A holds waves at q1, q2, field B at q3 = -(q1 + q2): only B in shell i and A in
+    # shells j, l contribute, so a swapped role of the two fields fails this
+    N, nb, kb, kc = 16, 5, 1.0, 1.0
+    q1, q2 = np.array([1, 2, 0]), np.array([-3, 0, 1])      # shells 2, 3
+    q3 = -(q1 + q2)                                          # shell 3
+    a = [0.7 + 0.2j, -0.3 + 0.9j, 1.1 - 0.4j]
+    fa = _plane_wave_field(N, [q1, q2], a[:2])
+    fb = _plane_wave_field(N, [q3], a[2:])
+    tc = O.triangle_list(nb, kb, kc, True, cross=True)
+    cc = O.count_triangles_exact(nb, N, kb, kc, True, cross=True).astype(np.float64) * N ** 3
+    kf = np.float32(2 * np.pi / 1000.0)
+    bout = O.measure_bk_cross(nb, N, kb, kc, True, kf, len(tc), 1, cc, fa, fb)
+    kf3 = O.kf3_bk(kf)
+    va = [(q1, a[0]), (q2, a[1])]
+    va = va + [(-v, np.conj(c)) for v, c in va]
+    vb = [(q3, a[2]), (-q3, np.conj(a[2]))]
+    for m, (l, j, i) in enumerate(tc):
+        s = 0.0
+        for vi, ci in vb:
+            for vj, cj in va:
+                for vl, cl in va:
+                    if np.any(vi + vj + vl):
+                        continue
+                    shells = tuple(O.shell_of_d2(v @ v, kb, kc) for v in (vi, vj, vl))
+                    if shells == (i, j, l):
+                        s += (ci * cj * cl).real
+        exp = kf3 * s * N ** 3 / cc[m]
+        assert np.isclose(bout[3, m], exp, rtol=1e-10, atol=1e-12 * abs(kf3 * N ** 3 / cc[m])), (l, j, i)
+    assert np.any(np.abs(bout[3]) > 0)
+
+
 def test_pow_parseval_and_pk_invariant():
     # pow(i) from the k maps equals half-space P0 - PSN on the same shells (kb = kc = 1 has no
     # bin-edge ties), up to the REAL(4) twopi vs 2*pi/box constants (~2e-6)
