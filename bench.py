@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--plain", type=int, default=3, help="catalogues for the plain path (0 = skip)")
     ap.add_argument("--latency-requests", type=int, default=24)
+    ap.add_argument("--mixed-requests", type=int, default=200,
+                    help="requests of the mixed-grid Poisson stream (0 = skip)")
+    ap.add_argument("--mixed-pool", type=int, default=24, help="distinct catalogues it cycles")
     ap.add_argument("--roofline-kernel", default="pbi4_deposit")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--stages", type=int, default=1, help="print a per-stage breakdown")
@@ -181,7 +184,7 @@ def main():
         achieved = float("nan")
     roof.update({"kernel": args.roofline_kernel, "achieved": round(achieved, 2),
                  "frac": round(achieved / roof["peak"], 4),
-                 "traffic": pmc_traffic("k_deposit<4>") if args.roofline_kernel == "pbi4_deposit" else None,
+                 "traffic": pmc_traffic("k_deposit<4, 2>") if args.roofline_kernel == "pbi4_deposit" else None,
                  "avg_launch_ms": round(avg_ms, 5), "launches": launches,
                  "algorithmic_bytes_per_launch": alg_bytes})
 
@@ -262,6 +265,11 @@ def main():
         lat["arrival_rate_per_s"] = round(rate, 3)
         log(f"[bench] Poisson stream at {rate:.1f}/s: {lat}")
 
+    # ---------------------------------------------------------------- mixed-grid stream
+    mixed = None
+    if args.mixed_requests > 0:
+        mixed = mixed_stream(ctx, torch, args.mixed_requests, args.mixed_pool, box)
+
     # ---------------------------------------------------------------- triangle GEMM alone
     gemm = gemm_bench(ctx, torch) if args.gemm else None
 
@@ -286,6 +294,7 @@ def main():
         "plain_fp32_catalogues_per_s": None if plain_value is None else round(plain_value, 4),
         "speedup_vs_plain": None if plain_value is None else round(value / plain_value, 3),
         "latency": lat,
+        "mixed_stream": mixed,
         "split_bf16_vs_fp32_err": None if errs is None else
         {"max": errs[0], "median": errs[1], "p99": errs[2]},
         "stage_ms_rsd_field": stages,
@@ -293,6 +302,91 @@ def main():
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def mixed_stream(ctx, torch, nreq, npool, box):
+    """Serving a mixed stream: `nreq` requests cycling `npool` resident Zel'dovich catalogues
+    whose particle counts are log-uniform in [1e6, 5e7] (lattice n = round(count^(1/3))) with
+    amplitudes uniform in [1e4, 4e4]; each request draws its grid from {128, 192, 256, 384}^3 and
+    its assignment from CIC/TSC/PCS (interlaced) and asks for redshift-space P_l (l = 0, 2, 4)
+    plus the bispectrum with kbin = kcenter = 2 up to 32 k_F (16 shells).  Reported: capacity
+    (all requests queued at t = 0), p50/p99 latency of a Poisson stream at 80% of it, and the
+    single-slot FP32 path (full-grid shell maps, FP32 triangle sums) over the same requests."""
+    from pbi4_amd import _lib
+    from pbi4_amd.catalogue import zeldovich
+    from pbi4_amd.pipeline import Field, Measurement
+    from pbi4_amd.serving import Request, ServingEngine, latency_summary
+    rng = np.random.default_rng(2024)
+    pool = []
+    for i in range(npool):
+        n = int(round(math.exp(rng.uniform(math.log(1e6), math.log(5e7))) ** (1.0 / 3.0)))
+        amp = float(rng.uniform(1e4, 4e4))
+        pool.append(zeldovich(n, box=box, amp=amp, seed=7000 + i, device="cuda"))
+    torch.cuda.synchronize()
+    draws = [(i % npool, int(rng.choice((128, 192, 256, 384))), int(rng.choice((2, 3, 4))))
+             for i in range(nreq)]
+
+    def meas(grid, order, prec, full):
+        return Measurement(grid=grid, order=order, interlacing=True, box=box, kbin=2.0,
+                           kcenter=2.0, nbins_ps=0, nbins_bisp=16, iopen=True,
+                           fields=(Field(True, (0.0, 0.0, 1.0)),), precision=prec,
+                           maps_grid=grid if full else 0)
+
+    def requests(prec, full):
+        out = []
+        for rid, (c, grid, order) in enumerate(draws):
+            p, v = pool[c]
+            out.append(Request(rid, meas(grid, order, prec, full), p, v, arrival=0.0))
+        return out
+
+    def warm(eng, prec, full):
+        # plans (rocFFT plans, exact triangle counts) and one pass per shape, untimed
+        seen = {}
+        for c, grid, order in draws:
+            seen.setdefault((grid, order), c)
+        eng.run([Request(-1, meas(g, o, prec, full), *pool[c], arrival=0.0)
+                 for (g, o), c in seen.items()])
+        torch.cuda.synchronize()
+
+    eng = ServingEngine(ctx, slots=4)
+    warm(eng, _lib.PREC_BF16X3, False)
+    t0 = time.perf_counter()
+    eng.run(requests(_lib.PREC_BF16X3, False))
+    torch.cuda.synchronize()
+    cap = nreq / (time.perf_counter() - t0)
+    rate = 0.8 * cap
+    reqs = requests(_lib.PREC_BF16X3, False)
+    t = np.cumsum(np.random.default_rng(6).exponential(1.0 / rate, nreq))
+    for r, a in zip(reqs, t):
+        r.arrival = float(a)
+    t0 = time.perf_counter()
+    done = eng.run(reqs)
+    torch.cuda.synchronize()
+    poisson_rate = nreq / (time.perf_counter() - t0)
+    lat = latency_summary(done)
+    del eng
+    plain = ServingEngine(ctx, slots=1)
+    warm(plain, _lib.PREC_FP32, True)
+    t0 = time.perf_counter()
+    plain.run(requests(_lib.PREC_FP32, True))
+    torch.cuda.synchronize()
+    plain_rate = nreq / (time.perf_counter() - t0)
+    del plain
+    counts = [int(pool[c][0].shape[1]) for c, _, _ in draws]
+    res = {"requests": nreq, "distinct_catalogues": npool,
+           "npart_min": min(counts), "npart_max": max(counts),
+           "grids": {g: sum(1 for d in draws if d[1] == g) for g in (128, 192, 256, 384)},
+           "orders": {o: sum(1 for d in draws if d[2] == o) for o in (2, 3, 4)},
+           "capacity_catalogues_per_s": round(cap, 3),
+           "poisson_arrival_rate_per_s": round(rate, 3),
+           "poisson_served_per_s": round(poisson_rate, 3),
+           "latency": lat,
+           "plain_fp32_single_slot_catalogues_per_s": round(plain_rate, 3),
+           "speedup_vs_plain": round(cap / plain_rate, 3)}
+    log(f"[bench] mixed stream: {res}")
+    del pool
+    torch.cuda.empty_cache()
+    return res
 
 
 def gemm_bench(ctx, torch):

@@ -34,7 +34,17 @@ __device__ __forceinline__ void pair_of(int p, int &i, int &j) {
     i = p - jj * (jj + 1) / 2;
 }
 
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// (a, b) -> packed bf16 hi and lo with a = hi + lo + O(2^-16 |a|): one conversion for two values
+// (v_cvt_pk_bf16_f32), the hi halves widened by shifts, one more conversion for the residuals.
+__device__ __forceinline__ void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
+    const f32x2 v = {a, b};
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+    const f32x2 r = {a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u)};
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+}
 
 template <int CF>
 struct GemmTile {
@@ -112,16 +122,11 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *map
                 if (c < nb) *reinterpret_cast<float4 *>(tf + c * LDF + 4 * xv) = reg[v];
                 __bf16 *bhi = reinterpret_cast<__bf16 *>(base + sizeof(float) * nb * LDF);
                 __bf16 *blo = bhi + NCP * LDB;
-                const float vv[4] = {reg[v].x, reg[v].y, reg[v].z, reg[v].w};
-                bf16x4 h, l;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const __bf16 hb = (__bf16)vv[k];
-                    h[k] = hb;
-                    l[k] = (__bf16)(vv[k] - (float)hb);
-                }
-                *reinterpret_cast<bf16x4 *>(bhi + c * LDB + 4 * xv) = h;
-                *reinterpret_cast<bf16x4 *>(blo + c * LDB + 4 * xv) = l;
+                uint32_t h[2], l[2];
+                split2(reg[v].x, reg[v].y, h[0], l[0]);
+                split2(reg[v].z, reg[v].w, h[1], l[1]);
+                *reinterpret_cast<uint2 *>(bhi + c * LDB + 4 * xv) = make_uint2(h[0], h[1]);
+                *reinterpret_cast<uint2 *>(blo + c * LDB + 4 * xv) = make_uint2(l[0], l[1]);
             } else {
                 *reinterpret_cast<float4 *>(tf + c * LDF + 4 * xv) = reg[v];
             }
@@ -142,15 +147,18 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *map
                     const float4 *ai = reinterpret_cast<const float4 *>(tf + pi[r] * LDF + xo);
                     const float4 *aj = reinterpret_cast<const float4 *>(tf + pj[r] * LDF + xo);
                     const float4 a0 = ai[0], a1 = ai[1], b0 = aj[0], b1 = aj[1];
-                    const float prod[8] = {a0.x * b0.x, a0.y * b0.y, a0.z * b0.z, a0.w * b0.w,
-                                           a1.x * b1.x, a1.y * b1.y, a1.z * b1.z, a1.w * b1.w};
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const __bf16 h = (__bf16)prod[e];
-                        ahi[r][e] = h;
-                        alo[r][e] = (__bf16)(prod[e] - (float)h);
-                    }
+                    uint32_t h[4], l[4];
+                    split2(a0.x * b0.x, a0.y * b0.y, h[0], l[0]);
+                    split2(a0.z * b0.z, a0.w * b0.w, h[1], l[1]);
+                    split2(a1.x * b1.x, a1.y * b1.y, h[2], l[2]);
+                    split2(a1.z * b1.z, a1.w * b1.w, h[3], l[3]);
+                    ahi[r] = __builtin_bit_cast(bf16x8, h);
+                    alo[r] = __builtin_bit_cast(bf16x8, l);
                 }
+                // Wide tiles: raise the wave's priority over its MFMA block so that the other wave on
+                // the SIMD issues its split VALU work in the MFMA shadow (measured faster for CF >= 6,
+                // slower for the VALU-heavier CF = 3 tiles).
+                if (CF >= 6) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                 for (int c = 0; c < CF; ++c) {
                     const int col = c * 16 + (lane & 15);
@@ -163,6 +171,7 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *map
                         acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r], bh, acc[r][c], 0, 0, 0);
                     }
                 }
+                if (CF >= 6) __builtin_amdgcn_s_setprio(0);
             }
         } else {
 #pragma unroll
