@@ -97,6 +97,19 @@ int pbi4_fcomb(pbi4_ctx *ctx, int grid, int nintp, int interlacing, float box, c
 int pbi4_compute_qk(pbi4_ctx *ctx, int grid, int norder, const int *idx, const void *deltak,
                     void *out, void *stream);
 
+/* Lowest-frequency sub-array of a density stored at a larger grid: powerI4.reduce_array
+ * (powerI4.f90:380-407), as DENSITIES.read_density uses it (pbi4/pbi4.py:257-288) to bring a
+ * density saved at grid L to the requested grid N <= L.
+ * din: device [z][y][x] array of L x L x lx complex elements, lx >= N/2+1 (only x <= N/2 is read).
+ * dout: device [z][y][x] array of N x N x nx elements, N/2+1 <= nx <= N.
+ * in_bytes / out_bytes: 8 = complex float, 16 = complex double (16 -> 8 rounds to nearest).
+ * Output (x, y, z) with x <= N/2 copies source (x, m(y), m(z)), m(i) = i for i < N - N/2 and
+ * L - N + i above: the reference's store order, in which the negative-frequency source stays on
+ * the Nyquist index of an even N.  Output x > N/2 is zero (the reference's zero-filled
+ * intent(out) array). */
+int pbi4_reduce_array(pbi4_ctx *ctx, int L, int lx, int N, int nx, int in_bytes, int out_bytes,
+                      const void *din, void *dout, void *stream);
+
 /* ------------------------------------------------------------------ power spectrum
  * Replaces powerI4.measure_pk (powerI4.f90:510-594, multipoles = 0) and
  * powerI4.measure_pk_multipoles (powerI4.f90:596-693, multipoles = 1) together with

@@ -10,6 +10,7 @@ reference's operation order, Fortran NINT/INT/MOD semantics, twopi = REAL(4) 6.2
   assign_double_grid ....... powerI4.f90:142-249     (C: oracle_kernels.c orc_assign)
   compute_density .......... pbi4/pbi4.py:304-341    (scipy.fft.fftn + fcomb)
   fcomb .................... powerI4.f90:251-378     (vectorised; the 8 ordered stores kept)
+  reduce_array ............. powerI4.f90:380-407     (the 4 ordered stores kept)
   compute_q2k / compute_q4k  powerI4.f90:409-508
   compute_density_multipoles pbi4/pbi4.py:343-388   (literal ifftn/fftn path)
   measure_pk ............... powerI4.f90:510-594     (C: orc_measure_pk mode 0)
@@ -20,7 +21,7 @@ reference's operation order, Fortran NINT/INT/MOD semantics, twopi = REAL(4) 6.2
   count_triangles .......... bispectrumI4.f90:763-915 (FFT form) and an exact integer form
 
 How each function is pinned (checked by tests/test_oracle.py):
-  * assignment, fcomb, q2k/q4k, measure_pk(_multipoles): against the reference powerI4.f90
+  * assignment, fcomb, reduce_array, q2k/q4k, measure_pk(_multipoles): against the reference powerI4.f90
     itself, compiled into oracle/_ref by oracle/Makefile, plus closed forms (B-spline weights,
     mass conservation, window sinc^p).
   * shell index / mode counts / triangle enumeration / N_tr: against the reference's saved
@@ -272,6 +273,26 @@ def fcomb(grid, nintp, interlacing, box, fdcl):
                 D[xs, icb, ica] = D[xs, icb, ica] * cfac
     del icy
     return D
+
+
+def reduce_array(L, din, grid):
+    """powerI4.f90:380-407 literally: the four ordered stores per (iy, iz), vectorised over ix
+    (1-based Fortran indices shifted to 0-based).  din (L, L, L) indexed [ix, iy, iz]; returns
+    (grid, grid, grid) complex128, zero outside ix <= grid/2 (intent(out) is zero-filled)."""
+    din = np.asarray(din)
+    dout = np.zeros((grid, grid, grid), dtype=np.complex128)
+    h = grid // 2 + 1
+    for iz in range(1, grid // 2 + 2):
+        icz = (L - iz + 1) % L + 1
+        jcz = (grid - iz + 1) % grid + 1
+        for iy in range(1, grid // 2 + 2):
+            icy = (L - iy + 1) % L + 1
+            jcy = (grid - iy + 1) % grid + 1
+            dout[:h, iy - 1, iz - 1] = din[:h, iy - 1, iz - 1]
+            dout[:h, jcy - 1, iz - 1] = din[:h, icy - 1, iz - 1]
+            dout[:h, iy - 1, jcz - 1] = din[:h, iy - 1, icz - 1]
+            dout[:h, jcy - 1, jcz - 1] = din[:h, icy - 1, icz - 1]
+    return dout
 
 
 def compute_density(npart, dcl, grid, nintp, interlacing, box, workers=None):

@@ -63,6 +63,8 @@ SIGNATURES = {
                                      c_size_t, c_void_p]),
     "pbi4_fcomb": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_void_p, c_void_p]),
     "pbi4_compute_qk": (c_int, [c_void_p, c_int, c_int, P(c_int), c_void_p, c_void_p, c_void_p]),
+    "pbi4_reduce_array": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                  c_void_p, c_void_p]),
     "pbi4_pk_workspace": (c_size_t, [c_void_p, P(PkDesc)]),
     "pbi4_measure_pk": (c_int, [c_void_p, P(PkDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_size_t, c_void_p]),

@@ -122,6 +122,26 @@ class Context:
                                           _stream(stream)))
         return out
 
+    def reduce_array(self, din, L, N, nx=None, out_dtype=None, out=None, stream=None):
+        """powerI4.reduce_array on the device: din a [z][y][x] (L, L, lx) complex tensor with
+        lx >= N/2+1, result (N, N, nx) of out_dtype (default din's dtype; complex128 ->
+        complex64 rounds) with x > N/2 zero."""
+        _check_dev(din, "din")
+        if din.dtype not in (torch.complex64, torch.complex128) or din.dim() != 3:
+            raise Pbi4Error("din must be a 3-D complex64 or complex128 tensor")
+        if din.shape[0] != L or din.shape[1] != L:
+            raise Pbi4Error(f"din must be ({L}, {L}, lx), got {tuple(din.shape)}")
+        nx = N // 2 + 1 if nx is None else int(nx)
+        out_dtype = din.dtype if out_dtype is None else out_dtype
+        if out is None:
+            out = torch.empty((N, N, nx), dtype=out_dtype, device=din.device)
+        _check_dev(out, "out", out_dtype)
+        nbytes = {torch.complex64: 8, torch.complex128: 16}
+        self._check(lib().pbi4_reduce_array(self.h, int(L), int(din.shape[2]), int(N), nx,
+                                            nbytes[din.dtype], nbytes[out_dtype], _ptr(din),
+                                            _ptr(out), _stream(stream)))
+        return out
+
     # ------------------------------------------------------------- P(k)
     def pk(self, delta, grid, box, kbin, kcenter, nbins, multipoles=False, los=(0.0, 0.0, 1.0),
            delta_b=None, delta2=None, delta4=None, out=None, stream=None):

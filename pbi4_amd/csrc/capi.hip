@@ -615,6 +615,19 @@ int pbi4_compute_qk(pbi4_ctx *ctx, int grid, int norder, const int *idx, const v
     });
 }
 
+int pbi4_reduce_array(pbi4_ctx *ctx, int L, int lx, int N, int nx, int in_bytes, int out_bytes,
+                      const void *din, void *dout, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(din && dout && din != dout, PBI4_EINVAL, "din and dout must be distinct buffers");
+        PBI4_CHECK(N >= 1 && N <= L, PBI4_EINVAL, "the requested grid must not exceed the stored one");
+        PBI4_CHECK(lx >= N / 2 + 1 && lx <= L && nx >= N / 2 + 1 && nx <= N, PBI4_EINVAL,
+                   "x extents must cover x <= N/2");
+        PBI4_CHECK((in_bytes == 8 || in_bytes == 16) && (out_bytes == 8 || out_bytes == 16),
+                   PBI4_EINVAL, "element sizes are 8 (complex float) or 16 (complex double)");
+        launch_reduce_array(L, lx, N, nx, in_bytes, out_bytes, din, dout, (hipStream_t)stream);
+    });
+}
+
 size_t pbi4_pk_workspace(pbi4_ctx *ctx, const pbi4_pk_desc *d) {
     (void)ctx;
     BinPlan b = make_bin_plan(d->grid, d->nbins, 0, d->multipoles, d->los, d->box > 0 ? d->box : 1.0f);

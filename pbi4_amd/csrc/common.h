@@ -77,6 +77,8 @@ void launch_combine(int N, int interlacing, const float *wtab, float cf, const f
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
                        float2 *out, hipStream_t s);
 void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, hipStream_t s);
+void launch_reduce_array(int L, int lx, int N, int nx, int in_bytes, int out_bytes, const void *din,
+                         void *dout, hipStream_t s);
 
 struct BinPlan {
     int N;                    // grid

@@ -533,6 +533,61 @@ void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, cons
     PBI4_HIP(hipGetLastError());
 }
 
+// powerI4.f90:380-407 (reduce_array) as a gather: output (x', y', z') of the N grid reads the L
+// grid at x = x' and y = m(y'), z = m(z') with m(i) = i below N - N/2 and L - N + i above.  The
+// reference writes each output cell from up to four (iy, iz) sources in the order (iy, iz),
+// (jcy, iz), (iy, jcz), (jcy, jcz); on the Nyquist index of an even N the later, negative-
+// frequency source is the one that stays, which is what m gives.  Rows are contiguous in x, so
+// both sides are coalesced.  Output x > N/2 is zero (the reference's intent(out) array is zero-
+// filled).
+template <typename TI, typename TO>
+__device__ __forceinline__ TO convert_complex(TI v) {
+    TO o;
+    o.x = v.x;
+    o.y = v.y;
+    return o;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_reduce_array(int L, int lx, int N, int nx, const TI *din,
+                                                      TO *dout) {
+    const int64_t n = (int64_t)N * N * nx;
+    const int lo = N - N / 2;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(t % nx);
+        const int64_t r = t / nx;
+        const int y = (int)(r % N), z = (int)(r / N);
+        TO v;
+        v.x = 0;
+        v.y = 0;
+        if (x <= N / 2) {
+            const int sy = y < lo ? y : L - N + y, sz = z < lo ? z : L - N + z;
+            v = convert_complex<TI, TO>(din[((int64_t)sz * L + sy) * lx + x]);
+        }
+        dout[t] = v;
+    }
+}
+
+void launch_reduce_array(int L, int lx, int N, int nx, int in_bytes, int out_bytes, const void *din,
+                         void *dout, hipStream_t s) {
+    const int64_t n = (int64_t)N * N * nx;
+    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 256), 1 << 20);
+    if (in_bytes == 16 && out_bytes == 16)
+        hipLaunchKernelGGL((k_reduce_array<double2, double2>), dim3(blocks), dim3(256), 0, s, L, lx, N,
+                           nx, static_cast<const double2 *>(din), static_cast<double2 *>(dout));
+    else if (in_bytes == 16)
+        hipLaunchKernelGGL((k_reduce_array<double2, float2>), dim3(blocks), dim3(256), 0, s, L, lx, N,
+                           nx, static_cast<const double2 *>(din), static_cast<float2 *>(dout));
+    else if (out_bytes == 16)
+        hipLaunchKernelGGL((k_reduce_array<float2, double2>), dim3(blocks), dim3(256), 0, s, L, lx, N,
+                           nx, static_cast<const float2 *>(din), static_cast<double2 *>(dout));
+    else
+        hipLaunchKernelGGL((k_reduce_array<float2, float2>), dim3(blocks), dim3(256), 0, s, L, lx, N,
+                           nx, static_cast<const float2 *>(din), static_cast<float2 *>(dout));
+    PBI4_HIP(hipGetLastError());
+}
+
 void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, hipStream_t s) {
     const int64_t n = (int64_t)N * N * N;
     const int4 id = make_int4(idx[0], idx[1], norder == 4 ? idx[2] : 1, norder == 4 ? idx[3] : 1);

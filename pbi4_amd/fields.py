@@ -47,9 +47,12 @@ class DeviceDensity:
     """Output of compute_density: the ix <= N/2+1 half of the reference's dcl as a complex64
     [z][y][x'] half spectrum (the only half any later routine reads)."""
 
-    def __init__(self, delta, grid, order, interlacing, box, npart):
+    def __init__(self, delta, grid, order, interlacing, box, npart, upper_zero=False):
         self.delta, self.grid, self.order = delta, int(grid), int(order)
         self.interlacing, self.box, self.npart = bool(interlacing), float(box), npart
+        # a density reduced from a larger grid (read_density) has a zero x > N/2 half, as the
+        # reference's reduce_array output does
+        self.upper_zero = bool(upper_zero)
 
     def full(self):
         """The reference's full dcl (N, N, N) complex128, indexed [ix, iy, iz]."""
@@ -58,7 +61,9 @@ class DeviceDensity:
         low = self.delta.cpu().numpy().astype(np.complex128).transpose(2, 1, 0)   # [x][y][z]
         D = np.empty((N, N, N), dtype=np.complex128, order="F")
         D[:h] = low
-        if N > 2:
+        if self.upper_zero:
+            D[h:] = 0.0
+        elif N > 2:
             xs = np.arange(h, N)
             iy = (N - np.arange(N)) % N
             mirror = low[N - xs][:, iy][:, :, iy]

@@ -180,6 +180,49 @@ class DENSITIES:
         self.params = params
         self.verbose = verbose
 
+    def read_density(self, params=None):
+        """(npart, DeviceDensity) from the density file params['dfile'] written by save_density
+        (pbi4/pbi4.py:257-288): header [grid, npart, kF, kN] and the complex128 (G, G, G) array.
+        A file grid G above params['grid'] is reduced on the device to the lowest frequencies
+        (powerI4.reduce_array); a smaller one is an error, as in the reference."""
+        if params is None:
+            params = self.params
+        print("FILE: %s\n" % params["dfile"])
+        print("Reading density...\n")
+        ctx = _ctx(params)
+        N = int(params["grid"])
+        with np.load(params["dfile"], allow_pickle=False) as f:
+            header = np.asarray(f["header"])
+            print("Grid: %.0f" % header[0])
+            print("Number of elements: %.0f\n" % header[1])
+            G = int(header[0])
+            if G < N:
+                print("\nERROR: Density file has smaller grid size than requested. \n")
+                raise SystemExit
+            data = f["data"]
+            if data.shape != (G, G, G):
+                raise Pbi4Error(f"density data must be ({G}, {G}, {G}), got {data.shape}")
+            # only the x <= N/2 slab is read downstream; [z][y][x] on the device
+            slab = torch.from_numpy(np.ascontiguousarray(data[:N // 2 + 1].transpose(2, 1, 0),
+                                                         dtype=np.complex128)).to(ctx.device)
+        if G > N:
+            print("Current grid size: %d" % G)
+            print("Requested grid size: %d" % N)
+            print("reducing density array to internal grid...")
+        delta = ctx.reduce_array(slab, G, N, out_dtype=torch.complex64)
+        npart = header[1]
+        return npart, DeviceDensity(delta, N, int(params["interpolation"]), bool(params["interlacing"]),
+                                    float(params["box"]), npart, upper_zero=G > N)
+
+    def save_density(self, dcl, npart, params=None):
+        """pbi4/pbi4.py:390-396: np.savez_compressed(dfile, header=[grid, npart, kF, kN],
+        data=dcl) with dcl as the reference's complex128 (N, N, N) array."""
+        if params is None:
+            params = self.params
+        print("FILE: %s" % params["dfile"])
+        header = np.array([params["grid"], npart, params["kf"], params["kN"]])
+        np.savez_compressed(params["dfile"], header=header, data=np.asarray(dcl))
+
     def assign_grid(self, pos, params=None, weight=None):
         """(npart, DeviceGrid): assign_double_grid / assign_single_grid (pbi4/pbi4.py:290-302)."""
         if params is None:

@@ -82,17 +82,14 @@ def measure_pk_multipoles(grid, box, kbin, kcenter, nbins, dcl, dcl2, dcl4):
 
 def reduce_array(L, din, grid):
     """powerI4.f90:380-407: the grid^3 sub-array of an L^3 density keeping the lowest
-    frequencies (an index remapping, done on the device)."""
+    frequencies, as a complex128 (grid, grid, grid) Fortran-order array whose x > grid/2 half is
+    zero (the reference's zero-filled intent(out) array).  The copy runs on the device in FP64
+    (bit-exact); only the x <= grid/2 slab of din is sent."""
     ctx = default_context()
     L, grid = int(L), int(grid)
-    src = full_from_array(din, ctx.device)                  # [z][y][x], x fastest
-    h = grid // 2 + 1
-    iz = torch.arange(h, device=ctx.device)
-    icz = (L - iz) % L
-    jcz = (grid - iz) % grid
-    out = torch.zeros((grid, grid, grid), dtype=src.dtype, device=ctx.device)
-    x = slice(0, h)
-    for (zs, zd) in ((iz, iz), (icz, jcz)):
-        for (ys, yd) in ((iz, iz), (icz, jcz)):
-            out[zd[:, None], yd[None, :], x] = src[zs[:, None], ys[None, :], x]
-    return full_to_array(out)
+    a = np.asarray(din)
+    if a.shape != (L, L, L):
+        raise Pbi4Error(f"din must have shape ({L}, {L}, {L})")
+    slab = np.ascontiguousarray(a[:grid // 2 + 1].transpose(2, 1, 0), dtype=np.complex128)
+    out = ctx.reduce_array(torch.from_numpy(slab).to(ctx.device), L, grid, nx=grid)
+    return np.asfortranarray(out.cpu().numpy().transpose(2, 1, 0))

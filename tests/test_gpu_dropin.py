@@ -213,18 +213,55 @@ def test_f2py_power_functions():
     _pk_close(mine, ref, [0, 1, 2])
 
 
-def test_reduce_array_against_reference():
-    if not os.path.exists(REF_SO):
-        pytest.skip("oracle/_ref not built")
+@pytest.mark.parametrize("L,G", [(16, 8), (12, 12), (16, 10), (9, 5), (64, 24)])
+def test_reduce_array_bit_exact(L, G):
+    # the device copy in FP64 against the oracle restatement (pinned to the reference's own
+    # reduce_array) and, where oracle/_ref travels, the reference routine itself
     import pbi4_amd.powerI4 as P
-    ref = ctypes.CDLL(REF_SO)
-    L, G = 16, 8
-    rng = np.random.default_rng(3)
+    rng = np.random.default_rng(3 + L + G)
     din = np.asfortranarray(rng.standard_normal((L, L, L)) + 1j * rng.standard_normal((L, L, L)))
-    out = np.zeros((G, G, G), dtype=np.complex128, order="F")
-    ref.ref_reduce_array(L, din.ctypes.data_as(ctypes.c_void_p), G, out.ctypes.data_as(ctypes.c_void_p))
     mine = P.reduce_array(L, din, G)
-    assert np.allclose(mine, out, atol=1e-6)
+    assert mine.shape == (G, G, G) and mine.dtype == np.complex128
+    assert np.array_equal(mine, O.reduce_array(L, din, G))
+    if os.path.exists(REF_SO):
+        ref = ctypes.CDLL(REF_SO)
+        out = np.zeros((G, G, G), dtype=np.complex128, order="F")
+        ref.ref_reduce_array(L, din.ctypes.data_as(ctypes.c_void_p), G, out.ctypes.data_as(ctypes.c_void_p))
+        assert np.array_equal(mine, out)
+
+
+def test_density_file_round_trip(tmp_path):
+    # DENSITIES.save_density / read_density (pbi4/pbi4.py:257-288, :390-396): the same grid reads
+    # back the identical device field; a smaller grid is the device reduce_array of the saved
+    # array, whose P(k) matches the FP64 oracle on the oracle's reduction of that array
+    import pbi4_amd as pb
+    L, N, n = 32, 16, 50000
+    pos = (_clustered(n, 21)[0].T % 1000.0 / 1000.0).astype(np.float32)
+    base = {"box": 1000.0, "interpolation": 4, "interlacing": True, "kbin": 1.0, "kcenter": 1.0,
+            "iopen": True, "multipoles": False, "los": np.array([0.0, 0.0, 1.0]), "cross": False,
+            "dfile": str(tmp_path / "density.npz")}
+    pL = {**base, "grid": L, "kf": 2 * np.pi / 1000.0, "kN": np.pi / 1000.0 * L}
+    dens = pb.DENSITIES(pL)
+    npart, g = dens.assign_grid(pos)
+    d = dens.compute_density(npart, g)
+    dens.save_density(d, npart, pL)
+    with np.load(pL["dfile"], allow_pickle=False) as f:
+        assert np.array_equal(f["header"], [L, npart, pL["kf"], pL["kN"]])
+        saved = np.asarray(f["data"])
+    npart2, d_same = dens.read_density(pL)
+    assert npart2 == npart and torch.equal(d_same.delta, d.delta)
+    pN = {**base, "grid": N, "kf": 2 * np.pi / 1000.0, "kN": np.pi / 1000.0 * N}
+    pN.update({"nbins_ps": N // 2, "kvec": np.arange(1, N // 2 + 1) * pN["kf"]})
+    _, d_red = pb.DENSITIES(pN).read_density(pN)
+    red = O.reduce_array(L, saved, N)
+    assert np.array_equal(np.asarray(d_red), red.astype(np.complex64).astype(np.complex128))
+    ps = pb.MEASUREMENTS(pN).powerspectrum(npart, d_red)
+    ref = O.measure_pk(N, 1000.0, 1.0, 1.0, N // 2, red, red)
+    mine = np.array([ps["avgk"], ps["P0+PSN"], ps["Nmodes"]], dtype=np.float64)
+    assert np.array_equal(mine[2], ref[2].astype(np.float32))
+    _pk_close(mine, ref.astype(np.float32).astype(np.float64), [0, 1, 2])
+    with pytest.raises(SystemExit):
+        pb.DENSITIES({**base, "grid": 2 * L}).read_density()
 
 
 def test_f2py_bispectrum_functions():

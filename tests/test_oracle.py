@@ -227,6 +227,54 @@ def test_fcomb_against_reference(inter):
     assert np.allclose(mine, out, rtol=1e-13, atol=1e-9 * np.abs(out).max())
 
 
+@pytest.mark.parametrize("L,G", [(16, 8), (12, 12), (16, 10), (9, 5)])
+def test_reduce_array_against_reference(L, G):
+    ref = _ref()
+    rng = np.random.default_rng(L * 31 + G)
+    din = np.asfortranarray(rng.standard_normal((L, L, L)) + 1j * rng.standard_normal((L, L, L)))
+    out = np.zeros((G, G, G), dtype=np.complex128, order="F")
+    ref.ref_reduce_array(L, din.ctypes.data_as(ctypes.c_void_p), G, out.ctypes.data_as(ctypes.c_void_p))
+    assert np.array_equal(O.reduce_array(L, din, G), out)
+
+
+def test_reduce_array_identity_and_plane_waves():
+    # N = L keeps the x <= N/2 half unchanged; a field of plane waves with |k_i| < N/2 sampled on
+    # L^3 reduces to the same waves' DFT on N^3 (times (L/N)^3, the DFT sum length)
+    L, N = 24, 12
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((L, L, L)) + 1j * rng.standard_normal((L, L, L))
+    d = O.reduce_array(L, a, L)
+    assert np.array_equal(d[:L // 2 + 1], a[:L // 2 + 1]) and not d[L // 2 + 1:].any()
+
+    def field(n):
+        x = np.arange(n) / n
+        X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+        f = np.zeros((n, n, n))
+        for (kx, ky, kz), amp in (((1, 2, -3), 1.0), ((5, -5, 0), 0.5), ((2, 0, 5), -2.0)):
+            f += amp * np.cos(2 * np.pi * (kx * X + ky * Y + kz * Z) + 0.3 * kx)
+        return np.fft.fftn(f)
+
+    red = O.reduce_array(L, field(L), N) * (N / L) ** 3
+    exp = field(N)
+    h = N // 2 + 1
+    assert np.allclose(red[:h], exp[:h], atol=1e-9 * np.abs(exp).max())
+
+
+def test_reduce_array_nyquist_takes_the_negative_frequency():
+    # the reference's store order: on the Nyquist index of an even N the source at L - N/2
+    # (frequency -N/2) overwrites the one at N/2
+    L, N = 16, 8
+    a = np.zeros((L, L, L), dtype=np.complex128)
+    a[1, N // 2, 2] = 1.0
+    a[1, L - N // 2, 2] = 2.0
+    a[1, 3, N // 2] = 3.0
+    a[1, 3, L - N // 2] = 4.0
+    a[2, N // 2, N // 2] = 5.0
+    a[2, L - N // 2, L - N // 2] = 6.0
+    d = O.reduce_array(L, a, N)
+    assert d[1, N // 2, 2] == 2.0 and d[1, 3, N // 2] == 4.0 and d[2, N // 2, N // 2] == 6.0
+
+
 def test_fcomb_interlaced_closed_form_interior():
     # for real grids g1, g2: delta = (G1 + phi G2) / (2 kF3 W) away from self-conjugate planes
     N = 16
