@@ -97,6 +97,23 @@ int pbi4_fcomb(pbi4_ctx *ctx, int grid, int nintp, int interlacing, float box, c
 int pbi4_compute_qk(pbi4_ctx *ctx, int grid, int norder, const int *idx, const void *deltak,
                     void *out, void *stream);
 
+/* Catalogue front-end: the redshift-space shift, periodic wrap and normalisation of
+ * IO.read_catalogues (pbi4/pbi4.py:151-154, 162-163, 179-180, 213-216 and 241-243) on positions
+ * a host reader produced, with numpy's exact arithmetic.
+ * pos: device (n, 3) rows of float (pos_f64 = 0) or double (1), box units (Mpc/h).
+ * vel: device (n, 3) rows (vel_f64 as for pos), or NULL with rsd = PBI4_RSD_NONE.
+ * rsd: PBI4_RSD_NONE; PBI4_RSD_HZ: s = p + ((v los_d) / hz) (1 + z) (Gadget, Rockstar and
+ *      Quijote readers, zf = 1 + z); PBI4_RSD_VEL: s = p + v los_d (the Minerva reader);
+ *      the shift is FP64 and s is stored in the output type.
+ * out: device (3, n) of float (out_f64 = 0) or double (1): min(1, ((s + box) % box) / box) in the
+ *      output type, numpy's remainder and correctly rounded division. */
+#define PBI4_RSD_NONE 0
+#define PBI4_RSD_HZ 1
+#define PBI4_RSD_VEL 2
+int pbi4_catalogue_positions(pbi4_ctx *ctx, const void *pos, int pos_f64, const void *vel, int vel_f64,
+                             int64_t n, int rsd, const double los[3], double hz, double zf, double box,
+                             void *out, int out_f64, void *stream);
+
 /* Lowest-frequency sub-array of a density stored at a larger grid: powerI4.reduce_array
  * (powerI4.f90:380-407), as DENSITIES.read_density uses it (pbi4/pbi4.py:257-288) to bring a
  * density saved at grid L to the requested grid N <= L.

@@ -11,6 +11,7 @@ reference's operation order, Fortran NINT/INT/MOD semantics, twopi = REAL(4) 6.2
   compute_density .......... pbi4/pbi4.py:304-341    (scipy.fft.fftn + fcomb)
   fcomb .................... powerI4.f90:251-378     (vectorised; the 8 ordered stores kept)
   reduce_array ............. powerI4.f90:380-407     (the 4 ordered stores kept)
+  catalogue_positions ...... pbi4/pbi4.py:151-243    (the RSD shift, wrap and normalisation lines)
   compute_q2k / compute_q4k  powerI4.f90:409-508
   compute_density_multipoles pbi4/pbi4.py:343-388   (literal ifftn/fftn path)
   measure_pk ............... powerI4.f90:510-594     (C: orc_measure_pk mode 0)
@@ -31,6 +32,8 @@ How each function is pinned (checked by tests/test_oracle.py):
   * measure_bk(_multipoles): closed-form bispectrum of a three-plane-wave field, symmetry of
     the triple sum, and the literal FFT-based formula at small N.
   * compute_density_multipoles: against the closed form delta_l = L_l(mu) delta.
+  * catalogue_positions: the reference's numpy lines run as written (numpy is the arithmetic
+    being restated), plus the invariants 0 <= x <= 1 and x * box = position mod box.
 """
 import ctypes
 import os
@@ -273,6 +276,23 @@ def fcomb(grid, nintp, interlacing, box, fdcl):
                 D[xs, icb, ica] = D[xs, icb, ica] * cfac
     del icy
     return D
+
+
+def catalogue_positions(pos, vel, rsd, los, hz, z, box, object_rows=False):
+    """IO.read_catalogues' shift, wrap and normalisation, literally on numpy arrays: the
+    in-place shift pos[:] += vel * los / Hz * (1 + z) (rsd 1: pbi4/pbi4.py:153-154, 163-164,
+    211-212) or pos[:] += vel * los (rsd 2: :179-180), then :241-243.  pos, vel (n, 3);
+    object_rows reproduces the Quijote branch, whose positions live in an object array of
+    Python floats (:203-207), converted to float64 at the end as f2py does.  Returns (3, n)."""
+    pos = np.array(pos, dtype=object) if object_rows else np.array(pos)
+    if rsd == 1:
+        pos[:] += vel[:] * np.asarray(los) / hz * (1. + z)
+    elif rsd == 2:
+        pos[:] += vel[:] * np.asarray(los)
+    pos = pos.T
+    pos = (pos + box) % box
+    pos = np.minimum(1. - np.nextafter(0, 1), pos / box)
+    return np.asarray(pos, dtype=np.float64) if object_rows else pos
 
 
 def reduce_array(L, din, grid):

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libpbi4hip.so")
 PBI4_OK = 0
 PREC_FP32 = 0
 PREC_BF16X3 = 1
+RSD_NONE, RSD_HZ, RSD_VEL = 0, 1, 2
 POS_NORMALISED = 0
 POS_BOX = 1
 
@@ -63,6 +64,9 @@ SIGNATURES = {
                                      c_size_t, c_void_p]),
     "pbi4_fcomb": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_void_p, c_void_p]),
     "pbi4_compute_qk": (c_int, [c_void_p, c_int, c_int, P(c_int), c_void_p, c_void_p, c_void_p]),
+    "pbi4_catalogue_positions": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int64, c_int,
+                                         P(ctypes.c_double), ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, c_void_p, c_int, c_void_p]),
     "pbi4_reduce_array": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
                                   c_void_p, c_void_p]),
     "pbi4_pk_workspace": (c_size_t, [c_void_p, P(PkDesc)]),

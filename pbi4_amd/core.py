@@ -122,6 +122,33 @@ class Context:
                                           _stream(stream)))
         return out
 
+    def catalogue_positions(self, pos, vel=None, rsd=_lib.RSD_NONE, los=(0.0, 0.0, 1.0), hz=1.0,
+                            zf=1.0, box=1000.0, out_dtype=None, out=None, stream=None):
+        """Redshift-space shift, wrap and normalisation of IO.read_catalogues on the device:
+        pos, vel (n, 3) float32/float64 rows -> (3, n) positions in [0, 1] of out_dtype (default
+        pos's dtype); see pbi4_catalogue_positions for the arithmetic."""
+        _check_dev(pos, "pos")
+        if pos.dim() != 2 or pos.shape[1] != 3 or pos.dtype not in (torch.float32, torch.float64):
+            raise Pbi4Error("pos must be an (n, 3) float32 or float64 tensor")
+        if vel is not None:
+            _check_dev(vel, "vel")
+            if vel.shape != pos.shape or vel.dtype not in (torch.float32, torch.float64):
+                raise Pbi4Error("vel must be float32 or float64 with pos's shape")
+        if rsd != _lib.RSD_NONE and vel is None:
+            raise Pbi4Error("a redshift-space shift needs velocities")
+        out_dtype = pos.dtype if out_dtype is None else out_dtype
+        n = pos.shape[0]
+        if out is None:
+            out = torch.empty((3, n), dtype=out_dtype, device=pos.device)
+        _check_dev(out, "out", out_dtype)
+        l3 = (ctypes.c_double * 3)(*[float(x) for x in los])
+        f64 = torch.float64
+        self._check(lib().pbi4_catalogue_positions(
+            self.h, _ptr(pos), int(pos.dtype == f64), _ptr(vel), int(vel is not None and vel.dtype == f64),
+            int(n), int(rsd), l3, float(hz), float(zf), float(box), _ptr(out), int(out_dtype == f64),
+            _stream(stream)))
+        return out
+
     def reduce_array(self, din, L, N, nx=None, out_dtype=None, out=None, stream=None):
         """powerI4.reduce_array on the device: din a [z][y][x] (L, L, lx) complex tensor with
         lx >= N/2+1, result (N, N, nx) of out_dtype (default din's dtype; complex128 ->

@@ -719,4 +719,83 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
     PBI4_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------------------------
+// Catalogue front-end: the redshift-space shift, periodic wrap and normalisation that
+// IO.read_catalogues applies to the positions a reader returned (pbi4/pbi4.py:151-154, 162-163,
+// 179-180, 213-216 for the shift; :241-243 for the wrap), restated with numpy's exact arithmetic:
+//   shift  : RSD_HZ  s = p + ((v * los_d) / Hz) * (1 + z),  RSD_VEL  s = p + v * los_d,
+//            both in FP64 (the float64 los promotes the product), then stored in the output type;
+//   wrap   : (s + box) % box in the output type, numpy's remainder (fmod, then + box when the
+//            sign differs from box's; an exact zero is +0);
+//   scale  : / box, correctly rounded, then min(1, x) (1 - nextafter(0, 1) is 1.0 in FP64).
+// Input (n, 3) rows as the readers build them, output (3, n) as the reference returns them.
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T exact_remainder(T a, T b) {
+    // fmod is exact; on [0, 2b) it is one exact subtraction (Sterbenz), the common case
+    T r = (a >= T(0) && a < b + b) ? (a < b ? a : a - b) : fmod(a, b);
+    if (r != T(0)) {
+        if ((r < T(0)) != (b < T(0))) r += b;
+    } else {
+        r = T(0);
+    }
+    return r;
+}
+
+template <typename TP, typename TV, typename TO>
+__global__ __launch_bounds__(256) void k_catalogue_positions(const TP *pos, const TV *vel, int64_t n,
+                                                             int rsd, double los0, double los1,
+                                                             double los2, double hz, double zf,
+                                                             double box, TO *out) {
+#pragma clang fp contract(off)
+    const TO b = (TO)box;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double los = d == 0 ? los0 : d == 1 ? los1 : los2;
+            TO s;
+            if (rsd == PBI4_RSD_NONE) {
+                s = (TO)pos[3 * i + d];
+            } else {
+                double t = (double)vel[3 * i + d] * los;
+                if (rsd == PBI4_RSD_HZ) t = (t / hz) * zf;
+                s = (TO)((double)pos[3 * i + d] + t);
+            }
+            const TO w = exact_remainder<TO>(s + b, b);
+            const TO x = w / b;
+            out[d * n + i] = x > TO(1) ? TO(1) : x;
+        }
+    }
+}
+
+template <typename TP, typename TV, typename TO>
+void launch_catalogue_t(const void *pos, const void *vel, int64_t n, int rsd, const double *los,
+                        double hz, double zf, double box, void *out, hipStream_t s) {
+    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 256), 1 << 16);
+    hipLaunchKernelGGL((k_catalogue_positions<TP, TV, TO>), dim3(blocks), dim3(256), 0, s,
+                       static_cast<const TP *>(pos), static_cast<const TV *>(vel), n, rsd, los[0],
+                       los[1], los[2], hz, zf, box, static_cast<TO *>(out));
+}
+
+}  // namespace
+
+void launch_catalogue_positions(const void *pos, int pos_f64, const void *vel, int vel_f64,
+                                int64_t n, int rsd, const double *los, double hz, double zf,
+                                double box, void *out, int out_f64, hipStream_t s) {
+    const int key = pos_f64 * 4 + vel_f64 * 2 + out_f64;
+    switch (key) {
+    case 0: launch_catalogue_t<float, float, float>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    case 1: launch_catalogue_t<float, float, double>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    case 2: launch_catalogue_t<float, double, float>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    case 3: launch_catalogue_t<float, double, double>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    case 4: launch_catalogue_t<double, float, float>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    case 5: launch_catalogue_t<double, float, double>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    case 6: launch_catalogue_t<double, double, float>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    default: launch_catalogue_t<double, double, double>(pos, vel, n, rsd, los, hz, zf, box, out, s); break;
+    }
+    PBI4_HIP(hipGetLastError());
+}
+
 }  // namespace pbi4

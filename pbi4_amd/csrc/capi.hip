@@ -615,6 +615,21 @@ int pbi4_compute_qk(pbi4_ctx *ctx, int grid, int norder, const int *idx, const v
     });
 }
 
+int pbi4_catalogue_positions(pbi4_ctx *ctx, const void *pos, int pos_f64, const void *vel, int vel_f64,
+                             int64_t n, int rsd, const double los[3], double hz, double zf, double box,
+                             void *out, int out_f64, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(pos && out && los && n >= 0, PBI4_EINVAL, "NULL argument or negative count");
+        PBI4_CHECK(rsd == PBI4_RSD_NONE || rsd == PBI4_RSD_HZ || rsd == PBI4_RSD_VEL, PBI4_EINVAL,
+                   "unknown rsd mode");
+        PBI4_CHECK(rsd == PBI4_RSD_NONE || vel, PBI4_EINVAL, "a redshift-space shift needs vel");
+        PBI4_CHECK(box > 0.0 && (rsd != PBI4_RSD_HZ || hz != 0.0), PBI4_EINVAL, "box must be > 0, hz != 0");
+        if (n == 0) return;
+        launch_catalogue_positions(pos, pos_f64 != 0, vel, vel_f64 != 0, n, rsd, los, hz, zf, box,
+                                   out, out_f64 != 0, (hipStream_t)stream);
+    });
+}
+
 int pbi4_reduce_array(pbi4_ctx *ctx, int L, int lx, int N, int nx, int in_bytes, int out_bytes,
                       const void *din, void *dout, void *stream) {
     return guarded(ctx, [&] {

@@ -77,6 +77,9 @@ void launch_combine(int N, int interlacing, const float *wtab, float cf, const f
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
                        float2 *out, hipStream_t s);
 void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, hipStream_t s);
+void launch_catalogue_positions(const void *pos, int pos_f64, const void *vel, int vel_f64,
+                                int64_t n, int rsd, const double *los, double hz, double zf,
+                                double box, void *out, int out_f64, hipStream_t s);
 void launch_reduce_array(int L, int lx, int N, int nx, int in_bytes, int out_bytes, const void *din,
                          void *dout, hipStream_t s);
 

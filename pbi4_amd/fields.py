@@ -29,6 +29,41 @@ def window_1d(N, order):
     return w
 
 
+class DeviceCatalogue:
+    """Output of IO.read_catalogues: the normalised positions (3, npart) as a device tensor
+    (float32, or float64 where the reference computes in float64).  np.asarray gives the
+    reference's array, which is float64 (its final np.minimum with the float64 scalar
+    1 - nextafter(0, 1) promotes); shape, dtype, T, astype, len and indexing behave as on it."""
+
+    def __init__(self, pos):
+        self.pos = pos
+
+    @property
+    def shape(self):
+        return tuple(self.pos.shape)
+
+    @property
+    def dtype(self):
+        return np.dtype(np.float64)
+
+    @property
+    def T(self):
+        return np.asarray(self).T
+
+    def astype(self, dtype):
+        return np.asarray(self).astype(dtype)
+
+    def __len__(self):
+        return self.pos.shape[0]
+
+    def __getitem__(self, idx):
+        return np.asarray(self)[idx]
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.pos.cpu().numpy().astype(np.float64)
+        return a.astype(dtype) if dtype is not None else a
+
+
 class DeviceGrid:
     """Output of assign_grid: grid 1 (and the interlacing grid 2) as float32 [z][y][x]."""
 

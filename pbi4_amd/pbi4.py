@@ -11,10 +11,10 @@ Differences a user can see:
   * Optional config keys: "precision" ("fp32" default, or "bf16x3" for the split-BF16 triangle
     GEMM), "maps_grid" (0 = band-limited grid chosen automatically, or an explicit size such as
     the full grid), "device" (GPU index).
-  * Catalogue formats: "synthetic" (seeded Zel'dovich catalogue, keys zel_n, zel_amp, zel_seed)
-    and "npz" / "sancho" (positions under 'pos' as (N, 3) or (3, N), optional 'vel').  The
-    other readers of pbi4/pbi4.py:143-228 (Gadget, Rockstar, Minerva, Flagship, Quijote, Molino)
-    are not part of this package.
+  * IO.read_catalogues returns a DeviceCatalogue (np.asarray gives the reference's positions).
+    Formats: the reference's gadget (binary), eos, minerva, flagship, quijote, molino and sancho
+    readers (pbi4_amd.readers), plus "npz" (positions under 'pos', optional 'vel') and
+    "synthetic" (seeded Zel'dovich catalogue, keys zel_n, zel_amp, zel_seed).
 """
 import argparse
 import json
@@ -26,9 +26,9 @@ import numpy as np
 import torch
 
 from . import _lib
-from .catalogue import zeldovich
 from .core import Pbi4Error, default_context
-from .fields import DeviceDensity, DeviceGrid, DeviceMultipole, half_from_array
+from .fields import DeviceCatalogue, DeviceDensity, DeviceGrid, DeviceMultipole, half_from_array
+from .readers import read_catalogue
 
 
 class NumpyArrayEncoder(JSONEncoder):
@@ -131,46 +131,42 @@ class IO:
         return d
 
     def read_catalogues(self, params=None):
-        """Positions normalised to [0, 1) as a (3, npart) float32 array (pbi4/pbi4.py:121-245),
-        shifted along params['los'] by v (1+z)/H(z) when params['multipoles'] is set."""
+        """Normalised positions (3, npart) in [0, 1] as a DeviceCatalogue (pbi4/pbi4.py:121-245):
+        the file is parsed on the host (pbi4_amd.readers), then the redshift-space shift along
+        params['los'] (when params['multipoles'] and the format has one), the periodic wrap and
+        the division by the box run on the device with the reference's arithmetic."""
         if params is None:
             params = self.params
-        cat = params.get("cat")
-        box = params["box"]
-        vel = None
-        if cat == "synthetic":
-            n = int(params.get("zel_n", 128))
-            pos, vel = zeldovich(n, box=box, amp=float(params.get("zel_amp", 2.0e4)),
-                                 seed=int(params.get("zel_seed", 0)),
-                                 device="cuda" if torch.cuda.is_available() else "cpu")
-            pos = pos.cpu().numpy().T
-            vel = vel.cpu().numpy().T
-        elif cat in ("npz", "sancho"):
-            print("Reading %s catalogue.\n" % cat)
-            print("FILE: %s\n" % params["ifile"])
-            with np.load(params["ifile"], allow_pickle=False) as mycat:
-                pos = np.asarray(mycat["pos"], dtype=np.float32)
-                if "vel" in mycat.files:
-                    vel = np.asarray(mycat["vel"], dtype=np.float32)
-            if pos.shape[0] == 3 and pos.shape[1] != 3:
-                pos = pos.T
-                vel = vel.T if vel is not None else None
-        else:
-            print("catalogue format %r is not supported by pbi4_amd (synthetic, npz, sancho)" % cat)
-            raise SystemExit
-        npart = len(pos)
+        c = read_catalogue(params, self.verbose)
+        npart = c.npart
         if self.verbose:
-            print("Found %d particles." % npart)
-        if params["multipoles"] and vel is not None:
-            los = np.asarray(params["los"])
+            print("Found %d %s." % (npart, c.kind))
+            if c.kind != "particles":
+                print("Number density: %.2e (h/Mpc)^3" % (npart / params["box"] ** 3))
+        los = np.asarray(params["los"], dtype=np.float64)
+        if c.rsd != _lib.RSD_NONE:
             print("Displacing along the LOS: (%.1f,%.1f,%.1f)" % (los[0], los[1], los[2]))
-            pos = pos + vel * los / params["Hz"] * (1.0 + params["z"])
-            pos = pos.astype(np.float32)
-        pos = pos.T
-        pos = (pos + np.float32(box)) % np.float32(box)
-        pos = np.minimum(1.0 - np.nextafter(0, 1), pos / np.float32(box)).astype(np.float32)
+        ctx = _ctx(params)
+
+        def dev(a):
+            if isinstance(a, torch.Tensor):
+                t = a.to(ctx.device)
+            else:
+                a = np.asarray(a)
+                if a.dtype not in (np.float32, np.float64):
+                    a = a.astype(np.float64)
+                t = torch.from_numpy(np.ascontiguousarray(a)).to(ctx.device)
+            return t.contiguous()
+
+        pos = dev(c.pos)
+        vel = dev(c.vel) if c.rsd != _lib.RSD_NONE else None
+        out_dtype = pos.dtype if c.out_dtype is None else (
+            torch.float64 if np.dtype(c.out_dtype) == np.float64 else torch.float32)
+        norm = ctx.catalogue_positions(pos, vel, c.rsd, tuple(los), float(params.get("Hz", 1.0)),
+                                       1.0 + float(params.get("z", 0.0)), float(params["box"]),
+                                       out_dtype=out_dtype)
         print("Done.")
-        return pos
+        return DeviceCatalogue(norm)
 
 
 class DENSITIES:
@@ -232,7 +228,9 @@ class DENSITIES:
         print("Assigning %d particles on the grid..." % npart)
         if weight is not None and not np.all(np.asarray(weight) == 1.0):
             raise Pbi4Error("per-particle weights other than 1 are not supported")
-        if isinstance(pos, torch.Tensor):
+        if isinstance(pos, DeviceCatalogue):
+            p = pos.pos.to(device=ctx.device, dtype=torch.float32).contiguous()
+        elif isinstance(pos, torch.Tensor):
             p = pos.to(device=ctx.device, dtype=torch.float32).contiguous()
         else:
             p = torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float32)).to(ctx.device)

@@ -230,6 +230,71 @@ def test_reduce_array_bit_exact(L, G):
         assert np.array_equal(mine, out)
 
 
+@pytest.mark.parametrize("pdt,vdt,rsd,obj,box", [
+    (np.float32, np.float32, 0, False, 1000), (np.float32, np.float32, 1, False, 1000),
+    (np.float32, np.float32, 2, False, 2500.5), (np.float64, np.float32, 1, False, 1000),
+    (np.float64, np.float64, 0, False, 777.7), (np.float32, np.float32, 1, True, 1000),
+    (np.float32, np.float32, 0, True, 1000)])
+def test_catalogue_positions_bit_exact(pdt, vdt, rsd, obj, box):
+    # the device shift / wrap / normalisation against the reference's numpy lines, including
+    # positions far outside the box and the edges 0, -0, box, -box
+    from pbi4_amd.core import default_context
+    ctx = default_context()
+    rng = np.random.default_rng(int(box) + 10 * rsd + obj)
+    n = 20000
+    pos = ((rng.random((n, 3)) * 3.4 - 1.2) * box).astype(pdt)
+    pos[:8] = np.array([0.0, -0.0, box, -box, 2 * box, -2.5 * box, box * (1 - 1e-7), 1e-30])[:, None]
+    vel = (rng.standard_normal((n, 3)) * 400).astype(vdt)
+    los = np.array([0.0, 2.0, 1.0]) / 5.0
+    hz, z = 130.86252328302402, 0.5
+    ref = O.catalogue_positions(pos, vel, rsd, los, hz, z, box, object_rows=obj)
+    # the object-array rows hold Python floats: float64 arithmetic throughout
+    out_dtype = torch.float64 if obj or pdt == np.float64 else torch.float32
+    dev = ctx.catalogue_positions(torch.from_numpy(pos).cuda(), torch.from_numpy(vel).cuda() if rsd else None,
+                                  rsd, tuple(los), hz, 1.0 + z, float(box), out_dtype=out_dtype)
+    assert np.array_equal(dev.cpu().numpy().astype(np.float64), np.asarray(ref, dtype=np.float64))
+
+
+def test_read_catalogues_gadget_and_quijote(tmp_path):
+    # IO.read_catalogues end to end: host reader, device front-end, DeviceCatalogue as the
+    # reference's float64 (3, n) array; then the P(k) of that catalogue
+    import pbi4_amd as pb
+    from pbi4_amd import readers as R
+    from test_readers import _fof_file, _gadget_file
+    rng = np.random.default_rng(8)
+    n = 30000
+    p_all = (rng.random((n + 5, 3)) * 1000).astype(np.float32)
+    v_all = (rng.standard_normal((n + 5, 3)) * 300).astype(np.float32)
+    _gadget_file(str(tmp_path / "snap"), 2, "<", [5, n, 0, 0, 0, 0], p_all, v_all, time=2 / 3, redshift=0.5)
+    cfg = {"cat": "gadget", "ifile": str(tmp_path / "snap"), "box": 1000, "grid": 32.0,
+           "interpolation": 3, "interlacing": True, "kbin": 1.0, "kcenter": 1.0, "iopen": True,
+           "om": 0.3, "ol": 0.7, "z": 0.5, "multipoles": True, "los": [0.0, 1.0, 1.0],
+           "cross": False, "config_file": str(tmp_path / "c.json")}
+    (tmp_path / "c.json").write_text(json.dumps(cfg))
+    io = pb.IO(config_file=str(tmp_path / "c.json"))
+    params = io.read_params()
+    cat = io.read_catalogues()
+    p, v = R.read_gadget(str(tmp_path / "snap"), 1, velocities=True)
+    ref = O.catalogue_positions(p, v, 1, params["los"], params["Hz"], params["z"], params["box"])
+    assert np.asarray(cat).dtype == np.float64 and np.array_equal(np.asarray(cat), ref)
+    npart, g = pb.DENSITIES(params).assign_grid(cat)
+    assert npart == n
+    # Quijote FoF halos with a redshift-space shift: float64 positions (object-array branch)
+    d = tmp_path / "groups_003"
+    d.mkdir()
+    _fof_file(str(d / "group_tab_003.0"), rng.random(n).astype(np.float32), p_all[:n] * 1e3, v_all[:n], n, 1)
+    cfg.update({"cat": "quijote", "ifile": str(tmp_path)})
+    (tmp_path / "c.json").write_text(json.dumps(cfg))
+    io = pb.IO(config_file=str(tmp_path / "c.json"))
+    params = io.read_params()
+    cat = io.read_catalogues()
+    assert cat.pos.dtype == torch.float64
+    _, gp, gv = R.read_fof_groups(str(tmp_path), 3)
+    ref = O.catalogue_positions(gp / np.float32(1e3), gv * np.float32(1.5), 1, params["los"],
+                                params["Hz"], params["z"], params["box"], object_rows=True)
+    assert np.array_equal(np.asarray(cat), ref)
+
+
 def test_density_file_round_trip(tmp_path):
     # DENSITIES.save_density / read_density (pbi4/pbi4.py:257-288, :390-396): the same grid reads
     # back the identical device field; a smaller grid is the device reduce_array of the saved

@@ -66,22 +66,25 @@ def test_io_derived_params_and_config_rewrite(tmp_path):
     assert back["itot"] == 1156 and len(back["kvec"]) == 32
 
 
-def test_io_read_catalogue_normalisation_and_rsd(tmp_path):
+def test_io_reader_dispatch_npz(tmp_path):
+    # the host half of IO.read_catalogues: the npz reader hands the device front-end the raw
+    # positions, the velocities and the (1 + z) / H(z) shift rule when multipoles are asked for
+    # (the shift, wrap and normalisation themselves are checked bit-exactly on the GPU)
     import pbi4_amd as pb
+    from pbi4_amd._lib import RSD_HZ, RSD_NONE
+    from pbi4_amd.readers import read_catalogue
     rng = np.random.default_rng(0)
     pos = (rng.random((500, 3)) * 1000 - 10).astype(np.float32)      # some outside the box
     vel = (rng.standard_normal((500, 3)) * 300).astype(np.float32)
     np.savez(tmp_path / "cat.npz", pos=pos, vel=vel)
     io = pb.IO(config_file=_config(tmp_path))
     p = io.read_params()
-    out = io.read_catalogues()
-    assert out.shape == (3, 500) and out.dtype == np.float32
-    assert out.min() >= 0 and out.max() < 1
-    # the reference's arithmetic (pbi4/pbi4.py:154, 241-243)
-    s = (pos + vel * p["los"] / p["Hz"] * (1.0 + p["z"])).astype(np.float32).T
-    s = (s + np.float32(1000)) % np.float32(1000)
-    exp = np.minimum(1.0 - np.nextafter(0, 1), s / np.float32(1000))
-    assert np.array_equal(out, exp.astype(np.float32))
+    c = read_catalogue(p)
+    assert c.rsd == RSD_HZ and c.npart == 500 and c.out_dtype is None
+    assert np.array_equal(c.pos, pos) and np.array_equal(c.vel, vel)
+    np.savez(tmp_path / "cat.npz", pos=pos.T)                           # (3, n), no velocities
+    c = read_catalogue(p)
+    assert c.rsd == RSD_NONE and c.vel is None and np.array_equal(c.pos, pos)
 
 
 def test_pipeline_itot_matches_reference_loop():
@@ -128,3 +131,35 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, f"{ROOT}/bench.py", "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_script_command_line_and_config_write_back(tmp_path):
+    # `import pbi4 as pb` + IO(scriptmode=True): command-line values override the JSON config,
+    # derived parameters are added and the file is rewritten (pbi4/pbi4.py:26-77, 86-119)
+    import os
+    import subprocess
+    import sys
+    cfg = {"cat": "sancho", "box": 1000, "grid": 144.0, "interpolation": 4, "interlacing": True,
+           "kbin": 2.0, "kcenter": 2.0, "iopen": True, "om": 0.3, "ol": 0.7, "z": 0.5,
+           "multipoles": True, "los": [0.0, 0.0, 1.0], "cross": False, "config_file": "config.json"}
+    (tmp_path / "config.json").write_text(json.dumps(cfg))
+    script = ("import json, pbi4 as pb\n"
+              "io = pb.IO(config_file='config.json', verbose=False, scriptmode=True)\n"
+              "p = io.read_params()\n"
+              "print(json.dumps({k: p[k] for k in ('grid', 'kbin', 'kcenter', 'interlacing', "
+              "'nbins_ps', 'nbins_bisp', 'itot')}))\n"
+              "print(json.dumps([float(x) for x in p['los']]))\n")
+    (tmp_path / "run.py").write_text(script)
+    env = dict(os.environ, PYTHONPATH=ROOT)
+    out = subprocess.run([sys.executable, "run.py", "--kbin", "1.", "--kcenter", "1.", "--grid", "64",
+                          "--los", "0", "2", "1", "--no-interlacing"], cwd=tmp_path, env=env,
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().splitlines()
+    got = json.loads(lines[-2])
+    assert got == {"grid": 64.0, "kbin": 1.0, "kcenter": 1.0, "interlacing": False,
+                   "nbins_ps": 32, "nbins_bisp": 21, "itot": got["itot"]}
+    assert np.allclose(json.loads(lines[-1]), [0.0, 0.4, 0.2])
+    written = json.loads((tmp_path / "config.json").read_text())
+    assert written["grid"] == 64.0 and written["interlacing"] is False
+    assert written["itot"] == got["itot"] and len(written["kvec"]) == 32
