@@ -65,10 +65,11 @@ def pmc_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "hbm_traffic_256cube_pcs.txt")
     try:
         for line in open(path):
-            f = line.split()
-            if f and f[0] == kernel and len(f) >= 3:
-                return round((float(f[1]) + float(f[2])) * 1e6)
-    except OSError:
+            # fixed-width table: a 60-character kernel name column (names contain spaces)
+            f = line[60:].split()
+            if line[:60].strip() == kernel and len(f) >= 2:
+                return round((float(f[0]) + float(f[1])) * 1e6)
+    except (OSError, ValueError):
         pass
     return None
 

@@ -163,3 +163,13 @@ def test_script_command_line_and_config_write_back(tmp_path):
     written = json.loads((tmp_path / "config.json").read_text())
     assert written["grid"] == 64.0 and written["interlacing"] is False
     assert written["itot"] == got["itot"] and len(written["kvec"]) == 32
+
+
+def test_bench_finds_the_roofline_kernels_traffic():
+    # the committed rocprofv3 traffic summary names the deposit kernel the bench reports
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", f"{ROOT}/bench.py")
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    t = b.pmc_traffic("k_deposit<4, 2>")
+    assert t is not None and t > 335544320          # at least the algorithmic bytes per launch
