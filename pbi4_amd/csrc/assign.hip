@@ -26,7 +26,7 @@ namespace {
 constexpr int kHalo = 2;          // region cells below the tile origin
 constexpr int kMaxR = 16 + 4;     // T <= 16
 constexpr int kDepositThreads = 1024;
-constexpr int kSortThreads = 1024;
+constexpr int kSortThreads = 256;     // several work-groups per CU overlap their phases
 constexpr int kHistPer = 8;       // particles per thread per histogram pass (loads in flight)
 constexpr int64_t kLdsHistMax = 32768;   // tiles counted in LDS (u32) per work-group
 
@@ -250,10 +250,16 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
         }
         return;
     }
+    // LDS: the tile counters, then the list of tiles this round touched and its length; only
+    // the touched tiles are flushed and cleared (a round of spatially ordered particles touches
+    // a few tens of the thousands of tiles)
     const int64_t round = (int64_t)blockDim.x * kScatterPer;
+    uint32_t *touched = lhist + ntiles;
+    uint32_t *ntouched = touched + round;
+    for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
+    if (threadIdx.x == 0) *ntouched = 0;
+    __syncthreads();
     for (int64_t r0 = beg; r0 < end; r0 += round) {
-        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
-        __syncthreads();
         uint32_t pk[kScatterPer][3];
         int tl[kScatterPer];
         uint32_t rank[kScatterPer];
@@ -272,7 +278,10 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
             run_rank(tl[k], rr, tail);
             const int lane = threadIdx.x & 63;
             uint32_t base = 0;
-            if (tl[k] >= 0 && tail) base = atomicAdd(&lhist[tl[k]], (uint32_t)(rr + 1));
+            if (tl[k] >= 0 && tail) {
+                base = atomicAdd(&lhist[tl[k]], (uint32_t)(rr + 1));
+                if (base == 0) touched[atomicAdd(ntouched, 1u)] = (uint32_t)tl[k];
+            }
             // the tail hands the run's base to the head (non-tails push to lane + 1, never a
             // head, so every head receives exactly one value); members read it from the head
             const int dst = tail ? lane - rr : lane + 1;
@@ -280,9 +289,10 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
             rank[k] = (uint32_t)__shfl((int)at_head, lane - rr) + (uint32_t)rr;
         }
         __syncthreads();
-        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
-            uint32_t c = lhist[t];
-            if (c) lhist[t] = atomicAdd(&cursor[t], c);
+        const uint32_t nt = *ntouched;
+        for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) {
+            const uint32_t t = touched[i];
+            lhist[t] = atomicAdd(&cursor[t], lhist[t]);
         }
         __syncthreads();
 #pragma unroll
@@ -295,6 +305,9 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
                 sz[slot] = pk[k][2];
             }
         }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) lhist[touched[i]] = 0;
+        if (threadIdx.x == 0) *ntouched = 0;
         __syncthreads();
     }
 }
@@ -681,13 +694,15 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
 
     const int use_lds = p.ntiles <= kLdsHistMax;
     const size_t lds = use_lds ? (size_t)p.ntiles * sizeof(uint32_t) : 0;
-    if (lds > 64 * 1024) {
+    const size_t lds_scatter =
+        use_lds ? lds + ((size_t)kSortThreads * kScatterPer + 1) * sizeof(uint32_t) : 0;
+    if (lds > 64 * 1024)
         PBI4_HIP(hipFuncSetAttribute((const void *)k_tile_hist,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (lds_scatter > 64 * 1024)
         PBI4_HIP(hipFuncSetAttribute((const void *)k_tile_scatter,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
-    const int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(npart, 8192)));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_scatter));
+    const int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(2048, ceil_div(npart, 8192)));
 
     launch_fill_u32(counts, 0u, p.ntiles + 1, s);
     if (npart > 0)
@@ -695,7 +710,7 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
                            p.ntiles, pos, vel, npart, counts, use_lds);
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, p.ntiles, offsets, cursor);
     if (npart > 0)
-        hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(kSortThreads), lds, s, x, p.T, p.NT,
+        hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(kSortThreads), lds_scatter, s, x, p.T, p.NT,
                            p.ntiles, pos, vel, npart, cursor, sx, sy, sz, use_lds);
     {
         TimedScope ts(timer, "pbi4_deposit", s);
