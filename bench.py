@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--stages", type=int, default=1, help="print a per-stage breakdown")
     ap.add_argument("--gemm", type=int, default=1,
                     help="time the triangle GEMM alone at the 512^3 / 48-shell shape")
+    ap.add_argument("--large", type=int, default=0,
+                    help="measure the large single box with this many particles per side and grid "
+                         "(1024 for the full configuration; three lines of sight, 32 shells) "
+                         "instead of the serving workload, print it and exit")
     ap.add_argument("--gemm-only", type=int, default=0,
                     help="run only the triangle GEMM timing (for profiling), print it and exit")
     return ap.parse_args()
@@ -111,6 +115,9 @@ def main():
     ctx = Context(0)
     if args.gemm_only:
         print(json.dumps({"triangle_gemm": gemm_bench(ctx, torch)}), flush=True)
+        return 0
+    if args.large:
+        print(json.dumps(large_box(ctx, torch, n=args.large)), flush=True)
         return 0
     N, n = args.grid, args.lattice
     box = 1000.0
@@ -405,6 +412,106 @@ def mixed_stream(ctx, torch, nreq, npool, box):
     del pool
     torch.cuda.empty_cache()
     return res
+
+
+def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
+    """The large single-box configuration: n^3 Zel'dovich particles (float32 positions and
+    velocities, 26 GB at n = 1024), grid n^3, PCS with interlacing, redshift space along x, y and
+    z, P_l (l = 0, 2, 4, kbin = kcenter = 2) and the split-BF16x3 bispectrum over 32 shells of
+    dk = 2 k_F.  Inputs resident in HBM; per line of sight the stages are timed with events on
+    the stream, the deposit and the triangle GEMM with the library's launch timers."""
+    from pbi4_amd import _lib
+    from pbi4_amd.catalogue import rsd_factor, zeldovich
+    from pbi4_amd.pipeline import CataloguePlan, Field, Measurement
+    N = n
+    log(f"[large] generating {n}^3 Zel'dovich particles on the GPU")
+    p, v = zeldovich(n, box=box, seed=4242, device="cuda")
+    torch.cuda.synchronize()
+    npart = n ** 3
+    los3 = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))
+    m = Measurement(grid=N, order=4, interlacing=True, box=box, kbin=2.0, kcenter=2.0,
+                    nbins_ps=0, nbins_bisp=32, iopen=True, fields=tuple(Field(True, l) for l in los3),
+                    precision=_lib.PREC_BF16X3, maps_grid=0)
+    t0 = time.perf_counter()
+    plan = CataloguePlan(ctx, m)
+    torch.cuda.synchronize()
+    log(f"[large] plan (bin tables, rocFFT plans, exact triangle counts) in "
+        f"{time.perf_counter() - t0:.2f} s: itot={plan.itot}, maps grid {plan.maps_grid}^3")
+    s = torch.cuda.current_stream()
+    coef = float(rsd_factor())
+
+    acc = {"dep": [0.0, 0], "gemm": [0.0, 0]}
+
+    def field(l, timed):
+        rsd = tuple(float(np.float32(coef * c)) for c in l)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        if timed:
+            ctx.timing_enable("pbi4_deposit")
+        ev[0].record(s)
+        g = ctx.assign(p, N, 4, True, vel=v, box=box, rsd=rsd, normalised=False)
+        ev[1].record(s)
+        if timed:
+            torch.cuda.synchronize()
+            ms, k = ctx.timing_read()
+            acc["dep"][0] += ms
+            acc["dep"][1] += k
+            ctx.timing_enable(None)
+        d = ctx.density(g, N, 4, True, box, npart)
+        del g
+        ev[2].record(s)
+        po = ctx.pk(d, N, box, 2.0, 2.0, m.nb_ps(), True, l)
+        ev[3].record(s)
+        desc = plan.desc[Field(True, l)]
+        desc.npart = npart
+        if timed:
+            ctx.timing_enable("pbi4_triangle_gemm")
+        bo = ctx.bk(d, plan.counts, desc)
+        ev[4].record(s)
+        torch.cuda.synchronize()
+        if timed:
+            ms, k = ctx.timing_read()
+            acc["gemm"][0] += ms
+            acc["gemm"][1] += k
+            ctx.timing_enable(None)
+        del d
+        return [ev[i].elapsed_time(ev[i + 1]) for i in range(4)], po, bo
+
+    field(los3[2], False)                           # warm-up (plans, workspaces)
+    t0 = time.perf_counter()
+    for rep in range(reps):
+        for l in los3:
+            field(l, False)
+    torch.cuda.synchronize()
+    per_cat = (time.perf_counter() - t0) / reps
+    stages = np.zeros(4)
+    for l in los3:                                  # a timed pass for the per-stage and kernel times
+        st, po, bo = field(l, True)
+        stages += np.array(st)
+    stages /= 3
+    dep_ms = acc["dep"][0] / max(acc["dep"][1], 1)
+    gemm_ms = acc["gemm"][0] / max(acc["gemm"][1], 1)
+    V = N ** 3
+    dep_gbs = (12.0 * npart + 8.0 * V) / (dep_ms * 1e-3) / 1e9
+    nb = m.nb_bisp()
+    gemm_tf = 3 * 2.0 * (nb * (nb + 1) // 2) * 3 * nb * plan.maps_grid ** 3 / (gemm_ms * 1e-3) / 1e12
+    out = {"metric": "large box: seconds per catalogue (3 lines of sight, P_l + B)",
+           "value": round(per_cat, 4), "unit": "s", "higher_is_better": False,
+           "config": {"workload": f"{n}^3 Zel'dovich particles, grid {N}^3 PCS interlaced, "
+                                  "redshift space along x, y, z, P_l l=0,2,4, B with 32 shells of "
+                                  "2 k_F (split-BF16x3)", "npart": npart, "itot": plan.itot,
+                      "maps_grid": plan.maps_grid},
+           "stage_ms_per_field": {k: round(float(x), 3) for k, x in
+                                  zip(("assign", "density", "pk", "bk"), stages)},
+           "deposit": {"avg_launch_ms": round(dep_ms, 3), "algorithmic_GBps": round(dep_gbs, 1),
+                       "frac_of_hbm": round(dep_gbs / HBM_PEAK_GBS, 4)},
+           "triangle_gemm": {"avg_launch_ms": round(gemm_ms, 3), "executed_tflops": round(gemm_tf, 1),
+                             "mfma_frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4)},
+           "pk_rows": int(po.shape[0]), "bk_rows": int(bo.shape[0]),
+           "finite": bool(torch.isfinite(po).all() and torch.isfinite(bo).all())}
+    log(f"[large] {out}")
+    del p, v
+    torch.cuda.empty_cache()
+    return out
 
 
 def gemm_bench(ctx, torch):

@@ -25,6 +25,10 @@ namespace {
 
 constexpr int kHalo = 2;          // region cells below the tile origin
 constexpr int kMaxR = 16 + 4;     // T <= 16
+// LDS plane stride of the deposit is R^2 + kPadZ words: with R^2 = 400 a z step of 4 cells is a
+// multiple of 64 words, so the particles of one atomic instruction (4 apart in tile order, which
+// for catalogues in lattice order means 4 cells apart along z) would all hit the same LDS banks
+constexpr int kPadZ = 9;
 constexpr int kDepositThreads = 1024;
 constexpr int kSortThreads = 256;     // several work-groups per CU overlap their phases
 constexpr int kHistPer = 8;       // particles per thread per histogram pass (loads in flight)
@@ -244,6 +248,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
             uint32_t pk[3];
             int t = tile_of(x, T, NT, load_particle(x, pos, vel, i, npart), pk);
             uint32_t slot = atomicAdd(&cursor[t], 1u);
+            if (slot >= (uint32_t)npart) continue;   // cannot happen; keeps a bad count in bounds
             sx[slot] = pk[0];
             sy[slot] = pk[1];
             sz[slot] = pk[2];
@@ -462,7 +467,7 @@ struct RegionCursor {
 
 // Deposits particles [c0, cend) of a tile into the LDS accumulators (one preload per thread).
 template <int ORDER, int NG>
-__device__ __forceinline__ void deposit_chunk(int *acc, int R, int R3, int per, float fix,
+__device__ __forceinline__ void deposit_chunk(int *acc, int R, int GS, int per, float fix,
                                               uint32_t c0, uint32_t cend, const uint32_t *sx,
                                               const uint32_t *sy, const uint32_t *sz) {
     constexpr int single = NG == 1;
@@ -504,12 +509,13 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R, int R3, int per, 
 #pragma unroll
                 for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
                 const float wx = pick4<ORDER>(w[0], ix) * fix;
-                int *a = acc + g * R3 + ((st[2] + kHalo) * R + st[1] + kHalo) * R + st[0] + kHalo + ix;
+                const int PZ = R * R + kPadZ;
+                int *a = acc + g * GS + (st[2] + kHalo) * PZ + (st[1] + kHalo) * R + st[0] + kHalo + ix;
 #pragma unroll
                 for (int iz = 0; iz < ORDER; ++iz)
 #pragma unroll
                     for (int iy = 0; iy < ORDER; ++iy)
-                        atomicAdd(a + (iz * R + iy) * R, __float2int_rn(wx * (w[1][iy] * w[2][iz])));
+                        atomicAdd(a + iz * PZ + iy * R, __float2int_rn(wx * (w[1][iy] * w[2][iz])));
             }
         }
     }
@@ -522,8 +528,9 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R,
                                                              const uint32_t *sy,
                                                              const uint32_t *sz,
                                                              float *regions) {
-    __shared__ int acc[NG * kMaxR * kMaxR * kMaxR];
+    __shared__ int acc[NG * kMaxR * (kMaxR * kMaxR + kPadZ)];
     const int R3 = R * R * R, ncell = NG * R3;
+    const int PZ = R * R + kPadZ, GS = R * PZ;        // padded LDS plane and grid strides
     constexpr int single = NG == 1;
     const int per = deposit_per_thread<ORDER>(single);
     const int64_t tile = blockIdx.x;
@@ -541,16 +548,16 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R,
     bool first = true;
     uint32_t r0 = b;
     do {
-        for (int i = threadIdx.x; i < ncell; i += blockDim.x) acc[i] = 0;
+        for (int i = threadIdx.x; i < NG * GS; i += blockDim.x) acc[i] = 0;
         const uint32_t rend = min(e, r0 + round);
         for (uint32_t c0 = r0; c0 < rend; c0 += chunk)
-            deposit_chunk<ORDER, NG>(acc, R, R3, per, fix, c0, min(rend, c0 + chunk), sx, sy, sz);
+            deposit_chunk<ORDER, NG>(acc, R, GS, per, fix, c0, min(rend, c0 + chunk), sx, sy, sz);
         __syncthreads();
         RegionCursor rc;
         rc.init(threadIdx.x, R);
         for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
             float *o = out + rc.g * R3 + region_index(rc.a, rc.b, rc.c, T, R);
-            const float v = (float)acc[i] * inv;
+            const float v = (float)acc[rc.g * GS + rc.c * PZ + rc.b * R + rc.a] * inv;
             *o = first ? v : *o + v;
             rc.advance(sa, sb, sc, R);
         }
