@@ -545,6 +545,8 @@ int pbi4_assign(pbi4_ctx *ctx, const pbi4_assign_desc *d, const float *pos, cons
                 int64_t npart, float *grids, void *ws, size_t ws_bytes, void *stream) {
     return guarded(ctx, [&] {
         PBI4_CHECK(d && grids && ws, PBI4_EINVAL, "NULL argument");
+        PBI4_CHECK(npart >= 0 && (npart == 0 || pos), PBI4_EINVAL,
+                   "positions missing or a negative particle count");
         AssignPlan p = make_assign_plan(*d);
         Arena a(ws, ws_bytes);
         std::lock_guard<std::mutex> g(ctx->mu);

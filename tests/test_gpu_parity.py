@@ -94,6 +94,75 @@ def test_assign_deterministic_and_box_mode(ctx):
     assert np.abs(_grid_to_xyz(a[0]) - ref.real).max() <= 2e-5 * scale
 
 
+@pytest.mark.parametrize("N", [20, 24, 36])
+def test_assign_and_density_tiles_other_than_16(ctx, N):
+    # tile sizes 10, 12, 12 (the largest divisor of N in [4, 16]): regions and their gather
+    n = 30000
+    pos = _pos(n, 300 + N)
+    for order, inter in ((4, 1), (3, 1), (2, 0), (1, 1)):
+        g = ctx.assign(_dev(pos), N, order, inter)
+        ref = (O.assign_double_grid if inter else O.assign_single_grid)(N, order, n, pos.astype(np.float64))
+        exp = ref.real if inter else ref
+        scale = np.abs(exp).max()
+        assert np.abs(_grid_to_xyz(g[0]) - exp).max() <= 2e-6 * scale, (order, inter)
+        if inter:
+            assert np.abs(_grid_to_xyz(g[1]) - ref.imag).max() <= 2e-6 * scale
+    delta = ctx.density(ctx.assign(_dev(pos), N, 4, True), N, 4, True, 1000.0, n)
+    _, dcl = _oracle_density(pos, N, 4, True)
+    exp = dcl[:N // 2 + 1]
+    assert np.abs(delta.cpu().numpy().transpose(2, 1, 0) - exp).max() <= 2e-5 * np.abs(exp).max()
+
+
+@pytest.mark.parametrize("order,inter", [(1, 1), (2, 0), (2, 1), (3, 1), (4, 0), (4, 1)])
+def test_assign_edge_positions(ctx, order, inter):
+    # coordinates at 0, 2^-24, 1 - 2^-24, exactly 1.0 (the reference's normalisation can return
+    # it; it wraps to cell 0), on cell and tile boundaries and one ulp either side of them
+    N = 32
+    vals = np.array([0.0, 2.0 ** -24, 1.0 - 2.0 ** -24, 1.0, 0.5, 1 / N, 15 / N, 16 / N, 17 / N,
+                     16 / N - 2.0 ** -24, 16 / N + 2.0 ** -23, 0.25 + 2.0 ** -20], dtype=np.float32)
+    g = np.stack(np.meshgrid(vals, vals, vals, indexing="ij")).reshape(3, -1)
+    pos = np.concatenate([g, _pos(2000, 77)], axis=1).astype(np.float32)
+    n = pos.shape[1]
+    out = ctx.assign(_dev(pos), N, order, inter)
+    ref = (O.assign_double_grid if inter else O.assign_single_grid)(N, order, n, pos.astype(np.float64))
+    exp = ref.real if inter else ref
+    scale = np.abs(exp).max()
+    # the lattice piles up to 64 particles near one corner: beside the FP32 term allow the
+    # fixed-point rounding of the deposit, at most 2^-19 per contribution (the LDS unit is never
+    # coarser than 2^-18), with order^3 x (particles per cell) contributions at most
+    per_cell = O.assign_single_grid(N, 1, n, pos.astype(np.float64)).max()
+    tol = 2e-6 * scale + 2.0 ** -19 * order ** 3 * per_cell
+    assert np.abs(_grid_to_xyz(out[0]) - exp).max() <= tol
+    if inter:
+        assert np.abs(_grid_to_xyz(out[1]) - ref.imag).max() <= tol
+
+
+def test_assign_empty_catalogue(ctx):
+    g = ctx.assign(torch.empty((3, 0), dtype=torch.float32, device="cuda"), 32, 4, True)
+    torch.cuda.synchronize()
+    assert g.shape == (2, 32, 32, 32) and not bool(g.any())
+
+
+@pytest.mark.parametrize("N", [512, 1024])
+def test_assign_large_tile_counts(ctx, N):
+    # 32768 tiles (the per-work-group LDS tile counters at their limit) and 262144 tiles (the
+    # global-atomic sort): mass conservation, and shifting every particle by exactly one cell
+    # along each axis rolls both grids by one cell (up to the per-tile fixed-point rounding)
+    n = 1 << 22
+    rng = np.random.default_rng(N)
+    q = rng.integers(0, 1 << 20, size=(3, n))                 # positions on a 2^-20 lattice
+    pos = (q * 2.0 ** -20).astype(np.float32)
+    shifted = ((q + (1 << 20) // N) % (1 << 20) * 2.0 ** -20).astype(np.float32)
+    a = ctx.assign(_dev(pos), N, 4, True)
+    sums = a.double().sum(dim=(1, 2, 3)).cpu().numpy()
+    assert np.allclose(sums, n, rtol=1e-5)
+    b = ctx.assign(_dev(shifted), N, 4, True)
+    rolled = torch.roll(a, shifts=(1, 1, 1), dims=(1, 2, 3))
+    assert float((b - rolled).abs().max()) <= 1e-5 * float(a.abs().max())
+    del a, b, rolled
+    torch.cuda.empty_cache()
+
+
 # ------------------------------------------------------------------ density
 
 
