@@ -32,7 +32,6 @@ constexpr int kPadZ = 9;
 constexpr int kDepositThreads = 1024;
 constexpr int kSortThreads = 256;     // several work-groups per CU overlap their phases
 constexpr int kHistPer = 8;       // particles per thread per histogram pass (loads in flight)
-constexpr int64_t kLdsHistMax = 32768;   // tiles counted in LDS (u32) per work-group
 
 struct PosXform {
     int N, pos_mode;
@@ -165,45 +164,96 @@ __device__ __forceinline__ void run_rank(int key, int &rank, bool &tail) {
     rank = lane - h;
 }
 
+// Tile counters of a work-group in LDS.  Direct: one counter per tile (ntiles <= kLdsDirectMax).
+// Hashed: kHashSlots (key, value) slots, open addressing from a multiplicative hash, keys set by
+// compare-and-swap; a run whose tile finds no slot within kHashProbes counts straight into global
+// memory.  A work-group of spatially ordered particles touches a few tens of tiles, so the table
+// stays sparse for any grid.
+constexpr int64_t kLdsDirectMax = 8192;
+constexpr int kHashSlots = 4096;
+constexpr int kHashProbes = 32;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int hash_slot(uint32_t *keys, uint32_t t, uint32_t *nkeys) {
+    uint32_t h = (t * 2654435761u) >> 20;                 // 12 bits: kHashSlots = 4096
+#pragma unroll 1
+    for (int probe = 0; probe < kHashProbes; ++probe) {
+        const uint32_t k = keys[h];
+        if (k == t) return (int)h;
+        if (k == kEmpty) {
+            const uint32_t prev = atomicCAS(&keys[h], kEmpty, t);
+            if (prev == kEmpty) {
+                if (nkeys) atomicAdd(nkeys, 1u);
+                return (int)h;
+            }
+            if (prev == t) return (int)h;
+        }
+        h = (h + 1) & (kHashSlots - 1);
+    }
+    return -1;
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, int NT,
                                                             int64_t ntiles, const float *pos,
                                                             const float *vel, int64_t npart,
-                                                            uint32_t *counts, int use_lds) {
+                                                            uint32_t *counts, int hashed) {
+    // hashed: values lhist[0..S), keys after them, then the number of keys in use; the table is
+    // flushed to global memory and cleared whenever it is a quarter full, so a catalogue in any
+    // order keeps its probes short
     extern __shared__ uint32_t lhist[];
+    uint32_t *keys = lhist + kHashSlots;
+    uint32_t *nkeys = lhist + 2 * kHashSlots;
+    const int64_t nctr = hashed ? kHashSlots : ntiles;
     const int64_t per_wg = ceil_div(npart, (int64_t)gridDim.x);
     const int64_t beg = (int64_t)blockIdx.x * per_wg;
     const int64_t end = min(npart, beg + per_wg);
-    if (use_lds) {
-        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
-        __syncthreads();
-        for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * kHistPer) {
-            RawParticle r[kHistPer];
+    for (int64_t t = threadIdx.x; t < nctr; t += blockDim.x) {
+        lhist[t] = 0;
+        if (hashed) keys[t] = kEmpty;
+    }
+    if (hashed && threadIdx.x == 0) *nkeys = 0;
+    __syncthreads();
+    for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * kHistPer) {
+        RawParticle r[kHistPer];
 #pragma unroll
-            for (int k = 0; k < kHistPer; ++k) {
-                const int64_t i = i0 + threadIdx.x + (int64_t)k * blockDim.x;
-                r[k] = load_particle(x, pos, vel, i < end ? i : beg, npart);
-            }
-            int t[kHistPer];
+        for (int k = 0; k < kHistPer; ++k) {
+            const int64_t i = i0 + threadIdx.x + (int64_t)k * blockDim.x;
+            r[k] = load_particle(x, pos, vel, i < end ? i : beg, npart);
+        }
+        int t[kHistPer];
 #pragma unroll
-            for (int k = 0; k < kHistPer; ++k) {
-                const int64_t i = i0 + threadIdx.x + (int64_t)k * blockDim.x;
-                t[k] = i < end ? tile_of(x, T, NT, r[k], nullptr) : -1;
-            }
+        for (int k = 0; k < kHistPer; ++k) {
+            const int64_t i = i0 + threadIdx.x + (int64_t)k * blockDim.x;
+            t[k] = i < end ? tile_of(x, T, NT, r[k], nullptr) : -1;
+        }
 #pragma unroll
-            for (int k = 0; k < kHistPer; ++k) {
-                int rank;
-                bool tail;
-                run_rank(t[k], rank, tail);
-                if (t[k] >= 0 && tail) atomicAdd(&lhist[t[k]], (uint32_t)(rank + 1));
+        for (int k = 0; k < kHistPer; ++k) {
+            int rank;
+            bool tail;
+            run_rank(t[k], rank, tail);
+            if (t[k] >= 0 && tail) {
+                const int slot = hashed ? hash_slot(keys, (uint32_t)t[k], nkeys) : t[k];
+                if (slot >= 0) atomicAdd(&lhist[slot], (uint32_t)(rank + 1));
+                else atomicAdd(&counts[t[k]], (uint32_t)(rank + 1));
             }
         }
-        __syncthreads();
-        for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x)
-            if (lhist[t]) atomicAdd(&counts[t], lhist[t]);
-    } else {
-        for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
-            atomicAdd(&counts[tile_of(x, T, NT, load_particle(x, pos, vel, i, npart), nullptr)], 1u);
+        if (hashed) {
+            __syncthreads();
+            if (*nkeys > kHashSlots / 4) {            // uniform: every thread reads one value
+                for (int t = threadIdx.x; t < kHashSlots; t += blockDim.x) {
+                    if (lhist[t]) atomicAdd(&counts[keys[t]], lhist[t]);
+                    lhist[t] = 0;
+                    keys[t] = kEmpty;
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) *nkeys = 0;
+            }
+            __syncthreads();
+        }
     }
+    __syncthreads();
+    for (int64_t t = threadIdx.x; t < nctr; t += blockDim.x)
+        if (lhist[t]) atomicAdd(&counts[hashed ? keys[t] : t], lhist[t]);
 }
 
 // Exclusive scan of the tile counts by one work-group; offsets[ntiles] = npart; cursor = offsets.
@@ -238,36 +288,30 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
                                                                const float *vel, int64_t npart,
                                                                uint32_t *cursor, uint32_t *sx,
                                                                uint32_t *sy, uint32_t *sz,
-                                                               int use_lds) {
+                                                               int hashed) {
+    // LDS: the round's counters (one per tile, or the hashed slots followed by their keys), then
+    // the list of counters this round touched and its length; only those are flushed and cleared
+    // (a round of spatially ordered particles touches a few tens of tiles)
     extern __shared__ uint32_t lhist[];
+    const int64_t nctr = hashed ? kHashSlots : ntiles;
+    uint32_t *keys = lhist + kHashSlots;
+    uint32_t *touched = lhist + (hashed ? 2 * kHashSlots : ntiles);
+    const int64_t round = (int64_t)blockDim.x * kScatterPer;
+    uint32_t *ntouched = touched + round;
     const int64_t per_wg = ceil_div(npart, (int64_t)gridDim.x);
     const int64_t beg = (int64_t)blockIdx.x * per_wg;
     const int64_t end = min(npart, beg + per_wg);
-    if (!use_lds) {
-        for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-            uint32_t pk[3];
-            int t = tile_of(x, T, NT, load_particle(x, pos, vel, i, npart), pk);
-            uint32_t slot = atomicAdd(&cursor[t], 1u);
-            if (slot >= (uint32_t)npart) continue;   // cannot happen; keeps a bad count in bounds
-            sx[slot] = pk[0];
-            sy[slot] = pk[1];
-            sz[slot] = pk[2];
-        }
-        return;
+    for (int64_t t = threadIdx.x; t < nctr; t += blockDim.x) {
+        lhist[t] = 0;
+        if (hashed) keys[t] = kEmpty;
     }
-    // LDS: the tile counters, then the list of tiles this round touched and its length; only
-    // the touched tiles are flushed and cleared (a round of spatially ordered particles touches
-    // a few tens of the thousands of tiles)
-    const int64_t round = (int64_t)blockDim.x * kScatterPer;
-    uint32_t *touched = lhist + ntiles;
-    uint32_t *ntouched = touched + round;
-    for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = 0;
     if (threadIdx.x == 0) *ntouched = 0;
     __syncthreads();
+    const int lane = threadIdx.x & 63;
     for (int64_t r0 = beg; r0 < end; r0 += round) {
         uint32_t pk[kScatterPer][3];
-        int tl[kScatterPer];
-        uint32_t rank[kScatterPer];
+        int ctr[kScatterPer];          // counter of the particle's run; -1: none; -2: direct
+        uint32_t rank[kScatterPer];    // rank in the round's counter, or the final slot if direct
         RawParticle raw[kScatterPer];
 #pragma unroll
         for (int k = 0; k < kScatterPer; ++k) {
@@ -277,41 +321,53 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
 #pragma unroll
         for (int k = 0; k < kScatterPer; ++k) {
             int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
-            tl[k] = i < end ? tile_of(x, T, NT, raw[k], pk[k]) : -1;
+            const int tl = i < end ? tile_of(x, T, NT, raw[k], pk[k]) : -1;
             int rr;
             bool tail;
-            run_rank(tl[k], rr, tail);
-            const int lane = threadIdx.x & 63;
+            run_rank(tl, rr, tail);
             uint32_t base = 0;
-            if (tl[k] >= 0 && tail) {
-                base = atomicAdd(&lhist[tl[k]], (uint32_t)(rr + 1));
-                if (base == 0) touched[atomicAdd(ntouched, 1u)] = (uint32_t)tl[k];
+            int c = -1;
+            if (tl >= 0 && tail) {
+                c = hashed ? hash_slot(keys, (uint32_t)tl, nullptr) : tl;
+                if (c >= 0) {
+                    base = atomicAdd(&lhist[c], (uint32_t)(rr + 1));
+                    if (base == 0) touched[atomicAdd(ntouched, 1u)] = (uint32_t)c;
+                } else {
+                    base = atomicAdd(&cursor[tl], (uint32_t)(rr + 1));   // final slots
+                    c = -2;
+                }
             }
-            // the tail hands the run's base to the head (non-tails push to lane + 1, never a
-            // head, so every head receives exactly one value); members read it from the head
+            // the tail hands the run's base and counter to the head (non-tails push to lane + 1,
+            // never a head, so every head receives exactly one value); members read them from
+            // the head
             const int dst = tail ? lane - rr : lane + 1;
-            const uint32_t at_head = (uint32_t)__builtin_amdgcn_ds_permute(4 * dst, (int)base);
-            rank[k] = (uint32_t)__shfl((int)at_head, lane - rr) + (uint32_t)rr;
+            const uint32_t b_head = (uint32_t)__builtin_amdgcn_ds_permute(4 * dst, (int)base);
+            const int c_head = __builtin_amdgcn_ds_permute(4 * dst, c);
+            rank[k] = (uint32_t)__shfl((int)b_head, lane - rr) + (uint32_t)rr;
+            ctr[k] = tl >= 0 ? __shfl(c_head, lane - rr) : -1;
         }
         __syncthreads();
         const uint32_t nt = *ntouched;
         for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) {
-            const uint32_t t = touched[i];
-            lhist[t] = atomicAdd(&cursor[t], lhist[t]);
+            const uint32_t c = touched[i];
+            lhist[c] = atomicAdd(&cursor[hashed ? keys[c] : c], lhist[c]);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kScatterPer; ++k) {
-            if (tl[k] >= 0) {
-                uint32_t slot = lhist[tl[k]] + rank[k];
-                if (slot >= (uint32_t)npart) continue;   // cannot happen; keeps a bad rank in bounds
-                sx[slot] = pk[k][0];
-                sy[slot] = pk[k][1];
-                sz[slot] = pk[k][2];
-            }
+            if (ctr[k] == -1) continue;
+            const uint32_t slot = ctr[k] >= 0 ? lhist[ctr[k]] + rank[k] : rank[k];
+            if (slot >= (uint32_t)npart) continue;   // cannot happen; keeps a bad rank in bounds
+            sx[slot] = pk[k][0];
+            sy[slot] = pk[k][1];
+            sz[slot] = pk[k][2];
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) lhist[touched[i]] = 0;
+        for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) {
+            const uint32_t c = touched[i];
+            lhist[c] = 0;
+            if (hashed) keys[c] = kEmpty;
+        }
         if (threadIdx.x == 0) *ntouched = 0;
         __syncthreads();
     }
@@ -699,10 +755,9 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
     for (int i = 0; i < 3; ++i) x.rsd[i] = p.rsd[i];
     x.has_vel = vel != nullptr && p.pos_mode == PBI4_POS_BOX;
 
-    const int use_lds = p.ntiles <= kLdsHistMax;
-    const size_t lds = use_lds ? (size_t)p.ntiles * sizeof(uint32_t) : 0;
-    const size_t lds_scatter =
-        use_lds ? lds + ((size_t)kSortThreads * kScatterPer + 1) * sizeof(uint32_t) : 0;
+    const int hashed = p.ntiles > kLdsDirectMax;
+    const size_t lds = (hashed ? 2 * (size_t)kHashSlots + 1 : (size_t)p.ntiles) * sizeof(uint32_t);
+    const size_t lds_scatter = lds + ((size_t)kSortThreads * kScatterPer + 1) * sizeof(uint32_t);
     if (lds > 64 * 1024)
         PBI4_HIP(hipFuncSetAttribute((const void *)k_tile_hist,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -714,11 +769,11 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
     launch_fill_u32(counts, 0u, p.ntiles + 1, s);
     if (npart > 0)
         hipLaunchKernelGGL(k_tile_hist, dim3(nwg), dim3(kSortThreads), lds, s, x, p.T, p.NT,
-                           p.ntiles, pos, vel, npart, counts, use_lds);
+                           p.ntiles, pos, vel, npart, counts, hashed);
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, p.ntiles, offsets, cursor);
     if (npart > 0)
         hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(kSortThreads), lds_scatter, s, x, p.T, p.NT,
-                           p.ntiles, pos, vel, npart, cursor, sx, sy, sz, use_lds);
+                           p.ntiles, pos, vel, npart, cursor, sx, sy, sz, hashed);
     {
         TimedScope ts(timer, "pbi4_deposit", s);
         auto launch = [&](auto kern) {

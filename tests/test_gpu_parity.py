@@ -113,6 +113,28 @@ def test_assign_and_density_tiles_other_than_16(ctx, N):
     assert np.abs(delta.cpu().numpy().transpose(2, 1, 0) - exp).max() <= 2e-5 * np.abs(exp).max()
 
 
+@pytest.mark.parametrize("N", [92, 148])
+@pytest.mark.parametrize("ordered", [False, True])
+def test_assign_hashed_tile_counters(ctx, N, ordered):
+    # 4-cell tiles: 23^3 = 12167 and 37^3 = 50653 tiles, counted in the per-work-group hash table;
+    # a random catalogue refills and flushes it every pass, a spatially ordered one makes long runs
+    n = 60000
+    pos = _pos(n, 900 + N)
+    if ordered:
+        pos = np.ascontiguousarray(pos[:, np.lexsort((pos[0], pos[1], pos[2]))])
+    per_cell = O.assign_single_grid(N, 1, n, pos.astype(np.float64)).max()
+    for order, inter in ((4, 1), (2, 0)):
+        g = ctx.assign(_dev(pos), N, order, inter)
+        ref = (O.assign_double_grid if inter else O.assign_single_grid)(N, order, n, pos.astype(np.float64))
+        exp = ref.real if inter else ref
+        # FP32 and the fixed-point rounding bound of the edge-position test; a particle lost or
+        # counted twice by the sort moves a cell by its weight, orders of magnitude more
+        tol = 2e-6 * np.abs(exp).max() + 2.0 ** -19 * order ** 3 * per_cell
+        assert np.abs(_grid_to_xyz(g[0]) - exp).max() <= tol, (order, inter)
+        if inter:
+            assert np.abs(_grid_to_xyz(g[1]) - ref.imag).max() <= tol
+
+
 @pytest.mark.parametrize("order,inter", [(1, 1), (2, 0), (2, 1), (3, 1), (4, 0), (4, 1)])
 def test_assign_edge_positions(ctx, order, inter):
     # coordinates at 0, 2^-24, 1 - 2^-24, exactly 1.0 (the reference's normalisation can return
@@ -145,8 +167,8 @@ def test_assign_empty_catalogue(ctx):
 
 @pytest.mark.parametrize("N", [512, 1024])
 def test_assign_large_tile_counts(ctx, N):
-    # 32768 tiles (the per-work-group LDS tile counters at their limit) and 262144 tiles (the
-    # global-atomic sort): mass conservation, and shifting every particle by exactly one cell
+    # 32768 and 262144 tiles (hashed tile counters, a random catalogue of 2^22 particles): mass
+    # conservation, and shifting every particle by exactly one cell
     # along each axis rolls both grids by one cell (up to the per-tile fixed-point rounding)
     n = 1 << 22
     rng = np.random.default_rng(N)
