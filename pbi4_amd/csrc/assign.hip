@@ -456,6 +456,54 @@ __device__ __forceinline__ void stencil(int grid_sel, int single, int c, float f
     }
 }
 
+// The stencil of stencil<ORDER> for ORDER 3 and 4 as its start and local coordinate h, and the
+// weight of stencil point k as a polynomial in h: w_k = a_k + b_k h + c_k h^2 + d_k h^3 (PCS: the
+// cubic B-spline pieces, h in [0, 1); TSC: the quadratic pieces, h in [-0.5, 0.5), point 3 zero).
+template <int ORDER>
+__device__ __forceinline__ void stencil_start(int grid_sel, int c, float f, int &start, float &h) {
+    if (ORDER == 4) {
+        if (grid_sel == 0) { start = c - 1; h = f; }
+        else if (f >= 0.5f) { start = c - 1; h = f - 0.5f; }
+        else { start = c - 2; h = f + 0.5f; }
+    } else {
+        if (grid_sel == 0) {
+            const int up = f >= 0.5f;
+            start = c - 1 + up;
+            h = f - (float)up;
+        } else {
+            start = c - 1;
+            h = f - 0.5f;
+        }
+    }
+}
+struct WeightPoly {
+    float a, b, c, d;
+};
+template <int ORDER>
+__device__ __forceinline__ WeightPoly weight_poly(int k) {
+    WeightPoly p;
+    if (ORDER == 4) {
+        p.a = k == 0 ? 1.0f / 6.0f : k == 1 ? 2.0f / 3.0f : k == 2 ? 1.0f / 6.0f : 0.0f;
+        p.b = k == 0 ? -0.5f : k == 2 ? 0.5f : 0.0f;
+        p.c = k == 0 ? 0.5f : k == 1 ? -1.0f : k == 2 ? 0.5f : 0.0f;
+        p.d = k == 0 ? -1.0f / 6.0f : k == 1 ? 0.5f : k == 2 ? -0.5f : 1.0f / 6.0f;
+    } else {
+        p.a = k == 0 ? 0.125f : k == 1 ? 0.75f : k == 2 ? 0.125f : 0.0f;
+        p.b = k == 0 ? -0.5f : k == 2 ? 0.5f : 0.0f;
+        p.c = k == 0 ? 0.5f : k == 1 ? -1.0f : k == 2 ? 0.5f : 0.0f;
+        p.d = 0.0f;
+    }
+    return p;
+}
+__device__ __forceinline__ float eval_poly(const WeightPoly &p, float h) {
+    return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(p.d, h, p.c), h, p.b), h, p.a);
+}
+// v of lane k of this lane's quad (DPP quad_perm broadcast)
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, true));
+}
+
 // LDS accumulation is 32-bit fixed point.  A chunk of kPer(ORDER) particles per thread at scale
 // 2^-s cannot overflow an LDS cell: the largest single contribution is (2/3)^3 for PCS, (3/4)^3
 // for TSC, 1 for CIC / NGP and 1.5^3 for the single-grid CIC of powerI4.f90:114-124
@@ -521,13 +569,23 @@ struct RegionCursor {
     }
 };
 
+// Nearest integer of a * b for |a * b| < 2^22, in one FMA and one integer subtraction: adding
+// 1.5 * 2^23 puts the exact product's rounding to an integer (ties to even) into the FMA's single
+// rounding, and the integer is the low mantissa bits.
+__device__ __forceinline__ int mul_rint(float a, float b) {
+    return __float_as_int(__builtin_fmaf(a, b, 12582912.0f)) - 0x4B400000;
+}
+
 // Deposits particles [c0, cend) of a tile into the LDS accumulators (one preload per thread).
-template <int ORDER, int NG>
-__device__ __forceinline__ void deposit_chunk(int *acc, int R, int GS, int per, float fix,
+// RC: the region edge R when known at compile time (T = 16), so the 16 row offsets of a lane's
+// atomics are instruction immediates; 0 = R at run time.
+template <int ORDER, int NG, int RC>
+__device__ __forceinline__ void deposit_chunk(int *acc, int R_, int GS, int per, float fix,
                                               uint32_t c0, uint32_t cend, const uint32_t *sx,
                                               const uint32_t *sy, const uint32_t *sz) {
     constexpr int single = NG == 1;
     constexpr int X = DepositLanes<ORDER>::X;
+    const int R = RC ? RC : R_;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ix = lane % X, j = lane / X;
     const uint32_t cb = c0 + (uint32_t)wave * 64u * (uint32_t)per;   // this wave's part
@@ -541,6 +599,9 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R, int GS, int per, 
         q[m][2] = ok ? sz[p] : 0u;
     }
     __syncthreads();   // the accumulators are cleared before the first chunk of a round
+    // orders 3 and 4: the four lanes of a particle are a quad; lane ix evaluates the weight of
+    // stencil point ix along each axis and the quad shares them by DPP broadcasts
+    const WeightPoly wp = weight_poly<ORDER>(ix);
 #pragma unroll
     for (int m = 0; m < kDepositPerMax; ++m) {
         if (m >= per || cb + 64u * m >= cend) continue;
@@ -550,6 +611,48 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R, int GS, int per, 
             uint32_t qq[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) qq[d] = (uint32_t)__shfl((int)q[m][d], src);
+            if constexpr (X == 4) {
+                // src is the same on the quad: the quad stays whole for the broadcasts
+                if (cb + (uint32_t)(64 * m + src) >= cend) continue;
+                int c[3];
+                float f[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    c[d] = (int)(qq[d] >> 28);
+                    f[d] = (float)(qq[d] & 0x0FFFFFFFu) * 3.7252902984619141e-09f;   // 2^-28
+                }
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    int st[3];
+                    float wl[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        float h;
+                        stencil_start<ORDER>(g, c[d], f[d], st[d], h);
+                        wl[d] = eval_poly(wp, h);
+                    }
+                    const float wx = wl[0] * fix;
+                    float wy[4], wxz[4];
+                    wy[0] = quad_bcast<0>(wl[1]);
+                    wy[1] = quad_bcast<1>(wl[1]);
+                    wy[2] = quad_bcast<2>(wl[1]);
+                    wy[3] = quad_bcast<3>(wl[1]);
+                    wxz[0] = wx * quad_bcast<0>(wl[2]);
+                    wxz[1] = wx * quad_bcast<1>(wl[2]);
+                    wxz[2] = wx * quad_bcast<2>(wl[2]);
+                    wxz[3] = wx * quad_bcast<3>(wl[2]);
+                    if (ix >= ORDER) continue;
+                    const int PZ = R * R + kPadZ;
+                    int *a = acc + g * GS + (st[2] + kHalo) * PZ + (st[1] + kHalo) * R + st[0] + kHalo + ix;
+                    // |w_x w_y w_z fix| <= fix <= 2^20: within mul_rint's range
+#pragma unroll
+                    for (int iz = 0; iz < ORDER; ++iz)
+#pragma unroll
+                        for (int iy = 0; iy < ORDER; ++iy)
+                            atomicAdd(a + iz * PZ + iy * R, mul_rint(wy[iy], wxz[iz]));
+                }
+                continue;
+            }
             if (cb + (uint32_t)(64 * m + src) >= cend || ix >= ORDER) continue;
             int c[3];
             float f[3];
@@ -565,26 +668,32 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R, int GS, int per, 
 #pragma unroll
                 for (int d = 0; d < 3; ++d) stencil<ORDER>(g, single, c[d], f[d], st[d], w[d]);
                 const float wx = pick4<ORDER>(w[0], ix) * fix;
+                float wxz[ORDER];
+#pragma unroll
+                for (int iz = 0; iz < ORDER; ++iz) wxz[iz] = wx * w[2][iz];
                 const int PZ = R * R + kPadZ;
                 int *a = acc + g * GS + (st[2] + kHalo) * PZ + (st[1] + kHalo) * R + st[0] + kHalo + ix;
+                // |w_x w_y w_z fix| <= max(1, 1.5^3 2^-2) 2^20 (single-grid CIC: weights in
+                // [-0.5, 1.5] at scale <= 2^18): within mul_rint's range
 #pragma unroll
                 for (int iz = 0; iz < ORDER; ++iz)
 #pragma unroll
                     for (int iy = 0; iy < ORDER; ++iy)
-                        atomicAdd(a + iz * PZ + iy * R, __float2int_rn(wx * (w[1][iy] * w[2][iz])));
+                        atomicAdd(a + iz * PZ + iy * R, mul_rint(w[1][iy], wxz[iz]));
             }
         }
     }
 }
 
-template <int ORDER, int NG>
-__global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R,
+template <int ORDER, int NG, int RC>
+__global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
                                                              const uint32_t *offsets,
                                                              const uint32_t *sx,
                                                              const uint32_t *sy,
                                                              const uint32_t *sz,
                                                              float *regions) {
     __shared__ int acc[NG * kMaxR * (kMaxR * kMaxR + kPadZ)];
+    const int R = RC ? RC : R_;
     const int R3 = R * R * R, ncell = NG * R3;
     const int PZ = R * R + kPadZ, GS = R * PZ;        // padded LDS plane and grid strides
     constexpr int single = NG == 1;
@@ -607,7 +716,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R,
         for (int i = threadIdx.x; i < NG * GS; i += blockDim.x) acc[i] = 0;
         const uint32_t rend = min(e, r0 + round);
         for (uint32_t c0 = r0; c0 < rend; c0 += chunk)
-            deposit_chunk<ORDER, NG>(acc, R, GS, per, fix, c0, min(rend, c0 + chunk), sx, sy, sz);
+            deposit_chunk<ORDER, NG, RC>(acc, R, GS, per, fix, c0, min(rend, c0 + chunk), sx, sy, sz);
         __syncthreads();
         RegionCursor rc;
         rc.init(threadIdx.x, R);
@@ -781,11 +890,20 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
                                p.R, offsets, sx, sy, sz, regions);
         };
         const bool two = p.ngrids == 2;
-        switch (p.order) {
-            case 1: two ? launch(k_deposit<1, 2>) : launch(k_deposit<1, 1>); break;
-            case 2: two ? launch(k_deposit<2, 2>) : launch(k_deposit<2, 1>); break;
-            case 3: two ? launch(k_deposit<3, 2>) : launch(k_deposit<3, 1>); break;
-            default: two ? launch(k_deposit<4, 2>) : launch(k_deposit<4, 1>); break;
+        if (p.R == kMaxR) {          // 16-cell tiles (any grid that 16 divides)
+            switch (p.order) {
+                case 1: two ? launch(k_deposit<1, 2, kMaxR>) : launch(k_deposit<1, 1, kMaxR>); break;
+                case 2: two ? launch(k_deposit<2, 2, kMaxR>) : launch(k_deposit<2, 1, kMaxR>); break;
+                case 3: two ? launch(k_deposit<3, 2, kMaxR>) : launch(k_deposit<3, 1, kMaxR>); break;
+                default: two ? launch(k_deposit<4, 2, kMaxR>) : launch(k_deposit<4, 1, kMaxR>); break;
+            }
+        } else {
+            switch (p.order) {
+                case 1: two ? launch(k_deposit<1, 2, 0>) : launch(k_deposit<1, 1, 0>); break;
+                case 2: two ? launch(k_deposit<2, 2, 0>) : launch(k_deposit<2, 1, 0>); break;
+                case 3: two ? launch(k_deposit<3, 2, 0>) : launch(k_deposit<3, 1, 0>); break;
+                default: two ? launch(k_deposit<4, 2, 0>) : launch(k_deposit<4, 1, 0>); break;
+            }
         }
     }
     {
