@@ -19,6 +19,8 @@
 //   NGP  cell nint(u)                                          (:225-233)
 #include "common.h"
 
+#include <type_traits>
+
 namespace pbi4 {
 
 namespace {
@@ -504,17 +506,18 @@ __device__ __forceinline__ float quad_bcast(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, true));
 }
 
-// LDS accumulation is 32-bit fixed point.  A chunk of kPer(ORDER) particles per thread at scale
-// 2^-s cannot overflow an LDS cell: the largest single contribution is (2/3)^3 for PCS, (3/4)^3
-// for TSC, 1 for CIC / NGP and 1.5^3 for the single-grid CIC of powerI4.f90:114-124
-// (h = vr - nint(vr) gives weights in [-0.5, 1.5]).  A tile of n particles deposits 2^k chunks
-// per round at scale 2^-(s-k), k = ceil(log2(n / chunk)), which keeps the same bound: dense tiles
-// use one round instead of many (each extra round re-reads and re-writes the region).  k stops
-// where the scale would fall below 2^-kMinScaleBits, which bounds the rounding error of one
-// contribution by 2^-19.  After each round the LDS tile is written to (first round) or added to
-// (later rounds, fixed order) the tile's region; integer sums within a round are order
-// independent, so the grids are bitwise reproducible.
-constexpr int kMinScaleBits = 18;
+// LDS accumulation is fixed point at 2^-s, s = 20 for TSC/PCS, 19 for CIC/NGP, 18 for the
+// single-grid CIC of powerI4.f90:114-124; each contribution is rounded to the nearest unit.  A cell
+// of the region receives contributions from the particles whose home cells lie in a window of
+// ORDER + 1 cells along each axis (either grid), at most B = min(n, (ORDER+1)^2 S) particles for a
+// tile of n particles whose fullest window of ORDER + 1 home cells along z holds S, each at most
+// w_max = (2/3)^3 (PCS), (3/4)^3 (TSC), 1 (CIC / NGP) or 1.5^3 (single-grid CIC, weights in
+// [-0.5, 1.5]) in magnitude.  If B (w_max 2^s + 1/2) < 2^32 (2^31 for the single-grid CIC, whose
+// sums can be negative) the tile accumulates in 32-bit LDS cells in one pass; otherwise in 64-bit
+// cells, one slab of the region at a time (a grid, or half of the single grid) in the same LDS,
+// each slab a pass over the tile's particles.  Either way a tile's cells are exact integer sums
+// over all its particles, independent of their order, so the grids are bitwise reproducible
+// whatever order the sort leaves the particles in.
 template <int ORDER>
 __device__ __forceinline__ int deposit_per_thread(int single) {
     return (ORDER >= 3) ? 4 : (ORDER == 2 && single) ? 1 : 2;
@@ -579,10 +582,22 @@ __device__ __forceinline__ int mul_rint(float a, float b) {
 // Deposits particles [c0, cend) of a tile into the LDS accumulators (one preload per thread).
 // RC: the region edge R when known at compile time (T = 16), so the 16 row offsets of a lane's
 // atomics are instruction immediates; 0 = R at run time.
-template <int ORDER, int NG, int RC>
-__device__ __forceinline__ void deposit_chunk(int *acc, int R_, int GS, int per, float fix,
-                                              uint32_t c0, uint32_t cend, const uint32_t *sx,
-                                              const uint32_t *sy, const uint32_t *sz) {
+__device__ __forceinline__ void lds_add(int *a, int v) { atomicAdd(a, v); }
+__device__ __forceinline__ void lds_add(unsigned long long *a, int v) {
+    atomicAdd(a, (unsigned long long)(long long)v);        // two's complement: signed add
+}
+
+// The part of the region one deposit pass accumulates: grid g, planes z in [zlo, zhi) (the
+// whole region when SLAB is false).
+struct Slab {
+    int g, zlo, zhi;
+};
+
+template <int ORDER, int NG, int RC, bool SLAB, typename Acc>
+__device__ __forceinline__ void deposit_chunk(Acc *acc, int R_, int GS, int per, float fix,
+                                              Slab sl, uint32_t c0, uint32_t cend,
+                                              const uint32_t *sx, const uint32_t *sy,
+                                              const uint32_t *sz) {
     constexpr int single = NG == 1;
     constexpr int X = DepositLanes<ORDER>::X;
     const int R = RC ? RC : R_;
@@ -623,6 +638,7 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R_, int GS, int per,
                 }
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
+                    if (SLAB && g != sl.g) continue;
                     int st[3];
                     float wl[3];
 #pragma unroll
@@ -643,13 +659,17 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R_, int GS, int per,
                     wxz[3] = wx * quad_bcast<3>(wl[2]);
                     if (ix >= ORDER) continue;
                     const int PZ = R * R + kPadZ;
-                    int *a = acc + g * GS + (st[2] + kHalo) * PZ + (st[1] + kHalo) * R + st[0] + kHalo + ix;
+                    const int z0 = st[2] + kHalo;
+                    const int base = (SLAB ? 0 : g * GS) + (st[1] + kHalo) * R + st[0] + kHalo + ix;
                     // |w_x w_y w_z fix| <= fix <= 2^20: within mul_rint's range
 #pragma unroll
-                    for (int iz = 0; iz < ORDER; ++iz)
+                    for (int iz = 0; iz < ORDER; ++iz) {
+                        if (SLAB && (z0 + iz < sl.zlo || z0 + iz >= sl.zhi)) continue;
+                        Acc *a = acc + base + (z0 + iz - (SLAB ? sl.zlo : 0)) * PZ;
 #pragma unroll
                         for (int iy = 0; iy < ORDER; ++iy)
-                            atomicAdd(a + iz * PZ + iy * R, mul_rint(wy[iy], wxz[iz]));
+                            lds_add(a + iy * R, mul_rint(wy[iy], wxz[iz]));
+                    }
                 }
                 continue;
             }
@@ -663,6 +683,7 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R_, int GS, int per,
             }
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
+                if (SLAB && g != sl.g) continue;
                 int st[3];
                 float w[3][4];
 #pragma unroll
@@ -672,19 +693,59 @@ __device__ __forceinline__ void deposit_chunk(int *acc, int R_, int GS, int per,
 #pragma unroll
                 for (int iz = 0; iz < ORDER; ++iz) wxz[iz] = wx * w[2][iz];
                 const int PZ = R * R + kPadZ;
-                int *a = acc + g * GS + (st[2] + kHalo) * PZ + (st[1] + kHalo) * R + st[0] + kHalo + ix;
+                const int z0 = st[2] + kHalo;
+                const int base = (SLAB ? 0 : g * GS) + (st[1] + kHalo) * R + st[0] + kHalo + ix;
                 // |w_x w_y w_z fix| <= max(1, 1.5^3 2^-2) 2^20 (single-grid CIC: weights in
                 // [-0.5, 1.5] at scale <= 2^18): within mul_rint's range
 #pragma unroll
-                for (int iz = 0; iz < ORDER; ++iz)
+                for (int iz = 0; iz < ORDER; ++iz) {
+                    if (SLAB && (z0 + iz < sl.zlo || z0 + iz >= sl.zhi)) continue;
+                    Acc *a = acc + base + (z0 + iz - (SLAB ? sl.zlo : 0)) * PZ;
 #pragma unroll
                     for (int iy = 0; iy < ORDER; ++iy)
-                        atomicAdd(a + iz * PZ + iy * R, mul_rint(w[1][iy], wxz[iz]));
+                        lds_add(a + iy * R, mul_rint(w[1][iy], wxz[iz]));
+                }
             }
         }
     }
 }
 
+// Writes the region of LDS accumulator `acc` (R^3 cells per grid at plane stride PZ) scaled by
+// `inv` into the tile's region `out` (27 pieces, region_index).
+template <typename Acc, typename Scale>
+__device__ __forceinline__ void flush_region(const Acc *acc, int T, int R, int PZ, int GS, int ncell,
+                                             Scale inv, float *out) {
+    const int R3 = R * R * R;
+    // flush cursor: this thread's first LDS cell and the stride blockDim.x in (a, b, c)
+    const int sc = blockDim.x / (R * R), sb = (blockDim.x / R) % R, sa = blockDim.x % R;
+    RegionCursor rc;
+    rc.init(threadIdx.x, R);
+    for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+        const Acc v = acc[rc.g * GS + rc.c * PZ + rc.b * R + rc.a];
+        out[rc.g * R3 + region_index(rc.a, rc.b, rc.c, T, R)] = (float)((Scale)v * inv);
+        rc.advance(sa, sb, sc, R);
+    }
+}
+
+template <int ORDER>
+__device__ __forceinline__ double deposit_wmax(int single) {
+    return ORDER == 4 ? 8.0 / 27.0 : ORDER == 3 ? 27.0 / 64.0 : (ORDER == 2 && single) ? 3.375 : 1.0;
+}
+
+// Largest value over the work-group (all threads get it; `red` holds blockDim.x / 64 ints).
+__device__ __forceinline__ int block_max(int v, int *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wave] = v;
+    __syncthreads();
+    v = red[0];
+    for (int w = 1; w < nw; ++w) v = max(v, red[w]);
+    return v;
+}
+
+// One work-group per tile: particles per home cell, the tile's accumulator width (above), the
+// LDS pass(es) over all its particles and the region write.
 template <int ORDER, int NG, int RC>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
                                                              const uint32_t *offsets,
@@ -692,7 +753,10 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
                                                              const uint32_t *sy,
                                                              const uint32_t *sz,
                                                              float *regions) {
-    __shared__ int acc[NG * kMaxR * (kMaxR * kMaxR + kPadZ)];
+    // 32-bit cells of NG grids, or 64-bit cells of one slab: NG kMaxR (kMaxR^2 + kPadZ) / 2 of
+    // them, a whole grid for NG = 2 and ceil(R / 2) planes for NG = 1
+    __shared__ __align__(16) int acc[NG * kMaxR * (kMaxR * kMaxR + kPadZ)];
+    __shared__ int red[kDepositThreads / 64];
     const int R = RC ? RC : R_;
     const int R3 = R * R * R, ncell = NG * R3;
     const int PZ = R * R + kPadZ, GS = R * PZ;        // padded LDS plane and grid strides
@@ -702,34 +766,66 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
     float *out = regions + tile * (int64_t)ncell;
     const uint32_t b = offsets[tile], e = offsets[tile + 1];
     const uint32_t chunk = (uint32_t)blockDim.x * per;      // particles per preload
-    const int bits = deposit_scale_bits<ORDER>(single);
-    int drop = 0;
-    while (bits - drop > kMinScaleBits && (chunk << drop) < e - b) ++drop;
-    const uint32_t round = chunk << drop;                    // particles per LDS round
-    const float fix = (float)(1 << (bits - drop));
-    const float inv = 1.0f / fix;
-    // flush cursor: this thread's first LDS cell and the stride blockDim.x in (a, b, c)
-    const int sc = blockDim.x / (R * R), sb = (blockDim.x / R) % R, sa = blockDim.x % R;
-    bool first = true;
-    uint32_t r0 = b;
-    do {
-        for (int i = threadIdx.x; i < NG * GS; i += blockDim.x) acc[i] = 0;
-        const uint32_t rend = min(e, r0 + round);
-        for (uint32_t c0 = r0; c0 < rend; c0 += chunk)
-            deposit_chunk<ORDER, NG, RC>(acc, R, GS, per, fix, c0, min(rend, c0 + chunk), sx, sy, sz);
+    const int sbits = deposit_scale_bits<ORDER>(single);
+    const float fix = (float)(1 << sbits);
+    // 32-bit cells hold B (w_max 2^s + 1/2) < 2^32 (weights >= 0; 2^31 for the single-grid CIC)
+    // for B contributions per cell: B = n always fits for small tiles; otherwise bound B by the
+    // fullest (ORDER+1)-cell window of home-cell counts along z times (ORDER+1)^2
+    constexpr bool sgn = ORDER == 2 && NG == 1;
+    constexpr int W = ORDER + 1;
+    const double lim = sgn ? 2147483648.0 : 4294967296.0;
+    const double per_contrib = deposit_wmax<ORDER>(single) * (double)fix + 0.5;
+    bool wide = false;
+    if ((double)(e - b) * per_contrib >= lim) {
+        const int T3 = T * T * T;                            // <= 4096 counters at acc[0..)
+        for (int i = threadIdx.x; i < T3; i += blockDim.x) acc[i] = 0;
         __syncthreads();
-        RegionCursor rc;
-        rc.init(threadIdx.x, R);
-        for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
-            float *o = out + rc.g * R3 + region_index(rc.a, rc.b, rc.c, T, R);
-            const float v = (float)acc[rc.g * GS + rc.c * PZ + rc.b * R + rc.a] * inv;
-            *o = first ? v : *o + v;
-            rc.advance(sa, sb, sc, R);
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x)
+            atomicAdd(&acc[((sz[i] >> 28) * T + (sy[i] >> 28)) * T + (sx[i] >> 28)], 1);
+        __syncthreads();
+        int best = 0;
+        if ((int)threadIdx.x < T * T) {                     // one (x, y) column per thread
+            const int col = threadIdx.x;
+            int win = 0;
+            for (int z = 0; z < T + W - 1; ++z) {
+                if (z < T) win += acc[z * T * T + col];
+                if (z >= W) win -= acc[(z - W) * T * T + col];
+                best = max(best, win);
+            }
         }
-        first = false;
-        r0 = rend;
+        best = block_max(best, red);
+        const double bound = fmin((double)(e - b), (double)(W * W) * best);
+        wide = bound * per_contrib >= lim;
+    }
+    __syncthreads();                                         // counters read before the clear
+    if (!wide) {
+        for (int i = threadIdx.x; i < NG * GS; i += blockDim.x) acc[i] = 0;
+        for (uint32_t c0 = b; c0 < e; c0 += chunk)
+            deposit_chunk<ORDER, NG, RC, false>(acc, R, GS, per, fix, Slab{0, 0, R}, c0,
+                                                min(e, c0 + chunk), sx, sy, sz);
         __syncthreads();
-    } while (r0 < e);
+        if (sgn) flush_region(acc, T, R, PZ, GS, ncell, 1.0f / fix, out);
+        else flush_region(reinterpret_cast<unsigned *>(acc), T, R, PZ, GS, ncell, 1.0f / fix, out);
+        return;
+    }
+    unsigned long long *acc64 = reinterpret_cast<unsigned long long *>(acc);
+    const double inv = 1.0 / (double)fix;
+    const int half = (R + 1) / 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        const Slab sl = NG == 2 ? Slab{pass, 0, R} : Slab{0, pass ? half : 0, pass ? R : half};
+        const int nsl = (sl.zhi - sl.zlo) * PZ;
+        for (int i = threadIdx.x; i < nsl; i += blockDim.x) acc64[i] = 0;
+        for (uint32_t c0 = b; c0 < e; c0 += chunk)
+            deposit_chunk<ORDER, NG, RC, true>(acc64, R, GS, per, fix, sl, c0, min(e, c0 + chunk),
+                                               sx, sy, sz);
+        __syncthreads();
+        for (int i = threadIdx.x; i < (sl.zhi - sl.zlo) * R * R; i += blockDim.x) {
+            const int c = i / (R * R), rem = i - c * (R * R), bb = rem / R, aa = rem - bb * R;
+            out[sl.g * R3 + region_index(aa, bb, sl.zlo + c, T, R)] =
+                (float)((double)(long long)acc64[c * PZ + bb * R + aa] * inv);
+        }
+        __syncthreads();
+    }
 }
 
 // grid[g][z][y][x] = sum of the region entries covering (x,y,z), in a fixed order: for each
@@ -796,6 +892,16 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
 }
 
 }  // namespace
+
+// Work-groups of kSortThreads threads with `lds` bytes of dynamic LDS that the device holds at
+// once (all compute units).
+static int resident_wgs(const void *kern, size_t lds) {
+    int dev = 0, cus = 0, per_cu = 0;
+    PBI4_HIP(hipGetDevice(&dev));
+    PBI4_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PBI4_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kSortThreads, lds));
+    return std::max(1, cus) * std::max(1, per_cu);
+}
 
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
@@ -867,44 +973,44 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
     const int hashed = p.ntiles > kLdsDirectMax;
     const size_t lds = (hashed ? 2 * (size_t)kHashSlots + 1 : (size_t)p.ntiles) * sizeof(uint32_t);
     const size_t lds_scatter = lds + ((size_t)kSortThreads * kScatterPer + 1) * sizeof(uint32_t);
-    if (lds > 64 * 1024)
-        PBI4_HIP(hipFuncSetAttribute((const void *)k_tile_hist,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (lds_scatter > 64 * 1024)
-        PBI4_HIP(hipFuncSetAttribute((const void *)k_tile_scatter,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_scatter));
     const int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(2048, ceil_div(npart, 8192)));
+    const int nwg_h = nwg, nwg_s = nwg;
 
-    launch_fill_u32(counts, 0u, p.ntiles + 1, s);
-    if (npart > 0)
-        hipLaunchKernelGGL(k_tile_hist, dim3(nwg), dim3(kSortThreads), lds, s, x, p.T, p.NT,
-                           p.ntiles, pos, vel, npart, counts, hashed);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, p.ntiles, offsets, cursor);
-    if (npart > 0)
-        hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(kSortThreads), lds_scatter, s, x, p.T, p.NT,
-                           p.ntiles, pos, vel, npart, cursor, sx, sy, sz, hashed);
+    {
+        TimedScope ts(timer, "pbi4_sort", s);
+        launch_fill_u32(counts, 0u, p.ntiles + 1, s);
+        if (npart > 0)
+            hipLaunchKernelGGL(k_tile_hist, dim3(nwg_h), dim3(kSortThreads), lds, s, x, p.T, p.NT,
+                               p.ntiles, pos, vel, npart, counts, hashed);
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, p.ntiles, offsets, cursor);
+        if (npart > 0)
+            hipLaunchKernelGGL(k_tile_scatter, dim3(nwg_s), dim3(kSortThreads), lds_scatter, s, x, p.T,
+                               p.NT, p.ntiles, pos, vel, npart, cursor, sx, sy, sz, hashed);
+    }
     {
         TimedScope ts(timer, "pbi4_deposit", s);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3((unsigned)p.ntiles), dim3(kDepositThreads), 0, s, p.T,
                                p.R, offsets, sx, sy, sz, regions);
         };
+#define PBI4_DEPOSIT(O, G, RC) launch(k_deposit<O, G, RC>)
         const bool two = p.ngrids == 2;
         if (p.R == kMaxR) {          // 16-cell tiles (any grid that 16 divides)
             switch (p.order) {
-                case 1: two ? launch(k_deposit<1, 2, kMaxR>) : launch(k_deposit<1, 1, kMaxR>); break;
-                case 2: two ? launch(k_deposit<2, 2, kMaxR>) : launch(k_deposit<2, 1, kMaxR>); break;
-                case 3: two ? launch(k_deposit<3, 2, kMaxR>) : launch(k_deposit<3, 1, kMaxR>); break;
-                default: two ? launch(k_deposit<4, 2, kMaxR>) : launch(k_deposit<4, 1, kMaxR>); break;
+                case 1: two ? PBI4_DEPOSIT(1, 2, kMaxR) : PBI4_DEPOSIT(1, 1, kMaxR); break;
+                case 2: two ? PBI4_DEPOSIT(2, 2, kMaxR) : PBI4_DEPOSIT(2, 1, kMaxR); break;
+                case 3: two ? PBI4_DEPOSIT(3, 2, kMaxR) : PBI4_DEPOSIT(3, 1, kMaxR); break;
+                default: two ? PBI4_DEPOSIT(4, 2, kMaxR) : PBI4_DEPOSIT(4, 1, kMaxR); break;
             }
         } else {
             switch (p.order) {
-                case 1: two ? launch(k_deposit<1, 2, 0>) : launch(k_deposit<1, 1, 0>); break;
-                case 2: two ? launch(k_deposit<2, 2, 0>) : launch(k_deposit<2, 1, 0>); break;
-                case 3: two ? launch(k_deposit<3, 2, 0>) : launch(k_deposit<3, 1, 0>); break;
-                default: two ? launch(k_deposit<4, 2, 0>) : launch(k_deposit<4, 1, 0>); break;
+                case 1: two ? PBI4_DEPOSIT(1, 2, 0) : PBI4_DEPOSIT(1, 1, 0); break;
+                case 2: two ? PBI4_DEPOSIT(2, 2, 0) : PBI4_DEPOSIT(2, 1, 0); break;
+                case 3: two ? PBI4_DEPOSIT(3, 2, 0) : PBI4_DEPOSIT(3, 1, 0); break;
+                default: two ? PBI4_DEPOSIT(4, 2, 0) : PBI4_DEPOSIT(4, 1, 0); break;
             }
         }
+#undef PBI4_DEPOSIT
     }
     {
         TimedScope ts(timer, "pbi4_gather", s);

@@ -135,6 +135,55 @@ def test_assign_hashed_tile_counters(ctx, N, ordered):
             assert np.abs(_grid_to_xyz(g[1]) - ref.imag).max() <= tol
 
 
+def _fixed_point_tolerance(pos, N, order, inter):
+    """Per-cell bound on the deposit's fixed-point rounding: half a unit (2^-20 for TSC/PCS, 2^-19
+    for CIC/NGP, 2^-18 for single-grid CIC, 32- or 64-bit cells alike) per contribution, one
+    contribution per particle whose home cell lies within order + 1 cells."""
+    smax = 20 if order >= 3 else (18 if order == 2 and not inter else 19)
+    home = np.floor(pos.astype(np.float64) * N).astype(np.int64) % N     # positions < 1
+    counts = np.zeros((N, N, N))
+    np.add.at(counts, (home[0], home[1], home[2]), 1)
+    box = np.zeros_like(counts)
+    r = order + 1
+    for dx in range(-r, r + 1):
+        for dy in range(-r, r + 1):
+            for dz in range(-r, r + 1):
+                box += np.roll(counts, (dx, dy, dz), (0, 1, 2))
+    return 0.5 * 2.0 ** -smax * box
+
+
+@pytest.mark.parametrize("case", ["clustered32", "clustered64", "pileup32"])
+@pytest.mark.parametrize("order,inter", [(4, 1), (3, 0), (2, 0), (1, 1)])
+def test_assign_dense_tiles_order_independent(ctx, case, order, inter):
+    # 2^20 particles: clustered on 32^3 and 64^3 grids (up to ~1500 particles in a cell: tiles in
+    # 64-bit LDS cells, alone or beside 32-bit ones), or half of them piled into one cell; the
+    # grids are bitwise the same for any order of the catalogue, conserve mass and match the oracle
+    n = 1 << 20
+    N = 64 if case == "clustered64" else 32
+    pos = _pos(n, 4242 + N)
+    if case == "pileup32":
+        rng = np.random.default_rng(5)
+        pos[:, : n // 2] = ((np.array([[9], [20], [3]]) + rng.random((3, n // 2))) / N).astype(np.float32)
+    perm = np.random.default_rng(N).permutation(n)
+    a = ctx.assign(_dev(pos), N, order, inter)
+    b = ctx.assign(_dev(pos[:, perm]), N, order, inter)
+    assert torch.equal(a, b)
+    # mass: NGP weights are 1, so its fixed-point grids are exact integers summing to n; other
+    # orders round each contribution to the nearest unit, which loses ~3e-7 n net (many
+    # contributions fall below half a unit)
+    tot = a.double().sum(dim=(1, 2, 3)).cpu().numpy()
+    if order == 1:
+        assert np.all(tot == n), tot
+    else:
+        assert np.abs(tot - n).max() <= 2e-6 * n, tot
+    ref = (O.assign_double_grid if inter else O.assign_single_grid)(N, order, n, pos.astype(np.float64))
+    exp = ref.real if inter else ref
+    tol = 2e-6 * np.abs(exp).max() + _fixed_point_tolerance(pos, N, order, inter)
+    assert np.all(np.abs(_grid_to_xyz(a[0]) - exp) <= tol)
+    if inter:
+        assert np.all(np.abs(_grid_to_xyz(a[1]) - ref.imag) <= tol)
+
+
 @pytest.mark.parametrize("order,inter", [(1, 1), (2, 0), (2, 1), (3, 1), (4, 0), (4, 1)])
 def test_assign_edge_positions(ctx, order, inter):
     # coordinates at 0, 2^-24, 1 - 2^-24, exactly 1.0 (the reference's normalisation can return
