@@ -32,6 +32,7 @@ constexpr int kMaxR = 16 + 4;     // T <= 16
 // for catalogues in lattice order means 4 cells apart along z) would all hit the same LDS banks
 constexpr int kPadZ = 9;
 constexpr int kDepositThreads = 1024;
+constexpr double kTileParticles = 8192.0;   // target mean particles per deposit tile
 constexpr int kSortThreads = 256;     // several work-groups per CU overlap their phases
 constexpr int kHistPer = 8;       // particles per thread per histogram pass (loads in flight)
 
@@ -907,7 +908,7 @@ void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
-AssignPlan make_assign_plan(const pbi4_assign_desc &d) {
+AssignPlan make_assign_plan(const pbi4_assign_desc &d, int64_t npart) {
     PBI4_CHECK(d.grid >= 4, PBI4_EINVAL, "grid must be >= 4");
     PBI4_CHECK(d.order >= 1 && d.order <= 4, PBI4_EINVAL, "interpolation order must be 1..4");
     PBI4_CHECK(d.interlacing == 0 || d.interlacing == 1, PBI4_EINVAL, "interlacing must be 0/1");
@@ -917,14 +918,22 @@ AssignPlan make_assign_plan(const pbi4_assign_desc &d) {
                "box must be > 0");
     AssignPlan p{};
     p.N = d.grid;
+    // tile edge: the largest divisor of N in [4, 16] whose tiles hold at most kTileParticles
+    // particles on average (one work-group deposits a tile, so crowded tiles serialise the
+    // deposit), but not below 8 cells while a divisor in [8, 16] exists (the region's 2 + 2 halo
+    // cells per axis cost (T+4)^3 / T^3 in region traffic)
     p.T = 0;
-    for (int t = 16; t >= 4; --t)
-        if (p.N % t == 0) {
-            p.T = t;
-            break;
-        }
-    PBI4_CHECK(p.T != 0, PBI4_EUNSUPPORTED,
+    int largest = 0, smallest8 = 0;
+    for (int t = 16; t >= 4; --t) {
+        if (p.N % t) continue;
+        if (!largest) largest = t;
+        if (t >= 8) smallest8 = t;
+        const double per_tile = (double)npart * t * t * t / ((double)p.N * p.N * p.N);
+        if (!p.T && per_tile <= kTileParticles && (t >= 8 || !smallest8)) p.T = t;
+    }
+    PBI4_CHECK(largest != 0, PBI4_EUNSUPPORTED,
                "grid " + std::to_string(p.N) + " has no divisor in [4, 16] for the tile size");
+    if (!p.T) p.T = smallest8 ? smallest8 : largest;
     p.NT = p.N / p.T;
     p.R = p.T + 2 * kHalo;
     p.ntiles = (int64_t)p.NT * p.NT * p.NT;
@@ -995,7 +1004,14 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
         };
 #define PBI4_DEPOSIT(O, G, RC) launch(k_deposit<O, G, RC>)
         const bool two = p.ngrids == 2;
-        if (p.R == kMaxR) {          // 16-cell tiles (any grid that 16 divides)
+        if (p.R == 12) {             // 8-cell tiles
+            switch (p.order) {
+                case 1: two ? PBI4_DEPOSIT(1, 2, 12) : PBI4_DEPOSIT(1, 1, 12); break;
+                case 2: two ? PBI4_DEPOSIT(2, 2, 12) : PBI4_DEPOSIT(2, 1, 12); break;
+                case 3: two ? PBI4_DEPOSIT(3, 2, 12) : PBI4_DEPOSIT(3, 1, 12); break;
+                default: two ? PBI4_DEPOSIT(4, 2, 12) : PBI4_DEPOSIT(4, 1, 12); break;
+            }
+        } else if (p.R == kMaxR) {   // 16-cell tiles
             switch (p.order) {
                 case 1: two ? PBI4_DEPOSIT(1, 2, kMaxR) : PBI4_DEPOSIT(1, 1, kMaxR); break;
                 case 2: two ? PBI4_DEPOSIT(2, 2, kMaxR) : PBI4_DEPOSIT(2, 1, kMaxR); break;

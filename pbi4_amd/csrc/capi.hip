@@ -535,7 +535,7 @@ const char *pbi4_last_error(const pbi4_ctx *ctx) { return ctx ? ctx->err.c_str()
 
 size_t pbi4_assign_workspace(const pbi4_assign_desc *d, int64_t npart) {
     try {
-        return assign_workspace(make_assign_plan(*d), npart);
+        return assign_workspace(make_assign_plan(*d, npart), npart);
     } catch (...) {
         return 0;
     }
@@ -547,7 +547,7 @@ int pbi4_assign(pbi4_ctx *ctx, const pbi4_assign_desc *d, const float *pos, cons
         PBI4_CHECK(d && grids && ws, PBI4_EINVAL, "NULL argument");
         PBI4_CHECK(npart >= 0 && (npart == 0 || pos), PBI4_EINVAL,
                    "positions missing or a negative particle count");
-        AssignPlan p = make_assign_plan(*d);
+        AssignPlan p = make_assign_plan(*d, npart);
         Arena a(ws, ws_bytes);
         std::lock_guard<std::mutex> g(ctx->mu);
         assign_run(p, pos, vel, npart, grids, a, (hipStream_t)stream, &ctx->timer);

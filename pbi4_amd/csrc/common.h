@@ -65,7 +65,7 @@ struct AssignPlan {
     int64_t ntiles;
     float box, rsd[3];
 };
-AssignPlan make_assign_plan(const pbi4_assign_desc &d);
+AssignPlan make_assign_plan(const pbi4_assign_desc &d, int64_t npart);
 size_t assign_workspace(const AssignPlan &p, int64_t npart);
 void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t npart,
                 float *grids, Arena &ws, hipStream_t s, KernelTimer *timer);
