@@ -292,11 +292,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
         }
     }
     __syncthreads();
-    float *out = partials + (int64_t)blockIdx.x * nq;
+    // partials are [quantity][work-group]: the finalize reads each quantity contiguously
     for (int i = threadIdx.x; i < nq; i += blockDim.x) {
         float s = acc[i];
         for (int w = 1; w < kBinWaves; ++w) s += acc[w * nq + i];
-        out[i] = s;
+        partials[(int64_t)i * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -309,7 +309,6 @@ __global__ __launch_bounds__(kFinThreads) void k_bin_finalize(BinPlan b, const f
                                                               double kF3, double *pk_out,
                                                               double *shell_sum) {
     __shared__ double red[5][kFinThreads];
-    const int nq = 5 * b.nbins_pk + b.nshell;
     const int i = blockIdx.x;
     const bool pk = i < b.nbins_pk;
     const int nv = pk ? 5 : 1;
@@ -320,7 +319,7 @@ __global__ __launch_bounds__(kFinThreads) void k_bin_finalize(BinPlan b, const f
     for (int g = threadIdx.x; g < b.nwg; g += kFinThreads) {
 #pragma unroll
         for (int q = 0; q < 5; ++q)
-            if (q < nv) s[q] += (double)partials[(int64_t)g * nq + col0 + q * cstride];
+            if (q < nv) s[q] += (double)partials[(int64_t)(col0 + q * cstride) * b.nwg + g];
     }
 #pragma unroll
     for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = s[q];
