@@ -207,7 +207,7 @@ def main():
         achieved = float("nan")
     roof.update({"kernel": args.roofline_kernel, "achieved": round(achieved, 2),
                  "frac": round(achieved / roof["peak"], 4),
-                 "traffic": pmc_traffic("k_deposit<4, 2>") if args.roofline_kernel == "pbi4_deposit" else None,
+                 "traffic": pmc_traffic("k_deposit<4, 2, 20>") if args.roofline_kernel == "pbi4_deposit" else None,
                  "avg_launch_ms": round(avg_ms, 5), "launches": launches,
                  "timing": "HIP events around each launch on its stream, one-slot timed pass of "
                            "the same workload",

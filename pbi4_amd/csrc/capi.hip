@@ -354,9 +354,10 @@ BinPlan make_bin_plan(int N, int nbins_pk, int nshell, int multipoles, const dou
     for (int i = 0; i < 3; ++i) b.los[i] = (float)los[i];
     b.kF = (float)(double)(kTwoPiF32 / box);
     const int64_t rows = (int64_t)N * N;
-    // 4 rows per work-group (one per wave) up to 4096 work-groups: the binning is bound by the
-    // latency of its dependent loads and DPP scans, so it wants every wave slot
-    b.nwg = (int)std::max<int64_t>(1, std::min<int64_t>(4096, ceil_div(rows, 4)));
+    // up to 2048 work-groups of 4 waves: the binning is bound by the latency of its dependent
+    // loads and DPP scans and wants every wave slot; each work-group adds one partial per bin to
+    // the finalize's reads
+    b.nwg = (int)std::max<int64_t>(1, std::min<int64_t>(2048, ceil_div(rows, 8)));
     return b;
 }
 

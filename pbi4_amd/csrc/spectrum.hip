@@ -316,6 +316,7 @@ __global__ __launch_bounds__(kFinThreads) void k_bin_finalize(BinPlan b, const f
     const int col0 = pk ? i : 4 * b.nbins_pk + i;
     const int cstride = pk ? b.nbins_pk : 0;
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int g = threadIdx.x; g < b.nwg; g += kFinThreads) {
 #pragma unroll
         for (int q = 0; q < 5; ++q)

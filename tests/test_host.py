@@ -171,5 +171,5 @@ def test_bench_finds_the_roofline_kernels_traffic():
     spec = importlib.util.spec_from_file_location("bench_mod", f"{ROOT}/bench.py")
     b = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(b)
-    t = b.pmc_traffic("k_deposit<4, 2>")
+    t = b.pmc_traffic("k_deposit<4, 2, 20>")
     assert t is not None and t > 335544320          # at least the algorithmic bytes per launch
