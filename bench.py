@@ -47,6 +47,12 @@ def parse():
     ap.add_argument("--mixed-requests", type=int, default=200,
                     help="requests of the mixed-grid Poisson stream (0 = skip)")
     ap.add_argument("--mixed-pool", type=int, default=24, help="distinct catalogues it cycles")
+    ap.add_argument("--mixed-shells", type=int, default=16,
+                    help="bispectrum shells of every mixed-stream request; 0 = the reference's "
+                         "default per grid (21, 32, 42, 64 at 128, 192, 256, 384)")
+    ap.add_argument("--mixed-slots", type=int, default=4, help="serving slots of the mixed stream")
+    ap.add_argument("--mixed-only", type=int, default=0,
+                    help="run only the mixed stream, print it and exit")
     ap.add_argument("--roofline-kernel", default="pbi4_deposit")
     ap.add_argument("--roofline-steps", type=int, default=4,
                     help="one-slot catalogues of the timed pass that times the roofline kernel")
@@ -119,6 +125,12 @@ def main():
     if args.large:
         print(json.dumps(large_box(ctx, torch, n=args.large)), flush=True)
         return 0
+    if args.mixed_only:
+        print(json.dumps({"mixed_stream": mixed_stream(ctx, torch, args.mixed_requests,
+                                                       args.mixed_pool, 1000.0,
+                                                       args.mixed_shells, args.mixed_slots)}),
+              flush=True)
+        return 0
     N, n = args.grid, args.lattice
     box = 1000.0
     m = Measurement(grid=N, order=4, interlacing=True, box=box, kbin=2.0, kcenter=2.0,
@@ -179,6 +191,7 @@ def main():
     torch.cuda.synchronize()
     kern_ms, launches = ctx.timing_read()
     ctx.timing_enable(None)
+    eng1.release()
     del eng1
 
     # ---------------------------------------------------------------- roofline of one kernel
@@ -292,8 +305,10 @@ def main():
 
     # ---------------------------------------------------------------- mixed-grid stream
     mixed = None
+    eng.release()
     if args.mixed_requests > 0:
-        mixed = mixed_stream(ctx, torch, args.mixed_requests, args.mixed_pool, box)
+        mixed = mixed_stream(ctx, torch, args.mixed_requests, args.mixed_pool, box,
+                             args.mixed_shells, args.mixed_slots)
 
     # ---------------------------------------------------------------- triangle GEMM alone
     gemm = gemm_bench(ctx, torch) if args.gemm else None
@@ -329,14 +344,15 @@ def main():
     return 0
 
 
-def mixed_stream(ctx, torch, nreq, npool, box):
+def mixed_stream(ctx, torch, nreq, npool, box, shells=16, slots=4):
     """Serving a mixed stream: `nreq` requests cycling `npool` resident Zel'dovich catalogues
     whose particle counts are log-uniform in [1e6, 5e7] (lattice n = round(count^(1/3))) with
     amplitudes uniform in [1e4, 4e4]; each request draws its grid from {128, 192, 256, 384}^3 and
     its assignment from CIC/TSC/PCS (interlaced) and asks for redshift-space P_l (l = 0, 2, 4)
-    plus the bispectrum with kbin = kcenter = 2 up to 32 k_F (16 shells).  Reported: capacity
-    (all requests queued at t = 0), p50/p99 latency of a Poisson stream at 80% of it, and the
-    single-slot FP32 path (full-grid shell maps, FP32 triangle sums) over the same requests."""
+    plus the bispectrum with kbin = kcenter = 2 over `shells` shells (16: up to 32 k_F; 0: the
+    reference's default count for the grid, N / (3 kbin)).  Reported: capacity (all requests
+    queued at t = 0), p50/p99 latency of a Poisson stream at 80% of it, and the single-slot FP32
+    path (full-grid shell maps, FP32 triangle sums) over the same requests."""
     from pbi4_amd import _lib
     from pbi4_amd.catalogue import zeldovich
     from pbi4_amd.pipeline import Field, Measurement
@@ -353,7 +369,7 @@ def mixed_stream(ctx, torch, nreq, npool, box):
 
     def meas(grid, order, prec, full):
         return Measurement(grid=grid, order=order, interlacing=True, box=box, kbin=2.0,
-                           kcenter=2.0, nbins_ps=0, nbins_bisp=16, iopen=True,
+                           kcenter=2.0, nbins_ps=0, nbins_bisp=shells, iopen=True,
                            fields=(Field(True, (0.0, 0.0, 1.0)),), precision=prec,
                            maps_grid=grid if full else 0)
 
@@ -373,12 +389,14 @@ def mixed_stream(ctx, torch, nreq, npool, box):
                  for (g, o), c in seen.items()])
         torch.cuda.synchronize()
 
-    eng = ServingEngine(ctx, slots=4)
+    eng = ServingEngine(ctx, slots=slots)
     warm(eng, _lib.PREC_BF16X3, False)
+    log("[bench] mixed stream: plans and one pass per shape done")
     t0 = time.perf_counter()
     eng.run(requests(_lib.PREC_BF16X3, False))
     torch.cuda.synchronize()
     cap = nreq / (time.perf_counter() - t0)
+    log(f"[bench] mixed stream: capacity {cap:.2f}/s")
     rate = 0.8 * cap
     reqs = requests(_lib.PREC_BF16X3, False)
     t = np.cumsum(np.random.default_rng(6).exponential(1.0 / rate, nreq))
@@ -389,16 +407,22 @@ def mixed_stream(ctx, torch, nreq, npool, box):
     torch.cuda.synchronize()
     poisson_rate = nreq / (time.perf_counter() - t0)
     lat = latency_summary(done)
+    eng.release()
     del eng
+    log(f"[bench] mixed stream: Poisson at {rate:.2f}/s served {poisson_rate:.2f}/s")
     plain = ServingEngine(ctx, slots=1)
     warm(plain, _lib.PREC_FP32, True)
+    log("[bench] mixed stream: FP32 path warmed")
     t0 = time.perf_counter()
     plain.run(requests(_lib.PREC_FP32, True))
     torch.cuda.synchronize()
     plain_rate = nreq / (time.perf_counter() - t0)
+    plain.release()
     del plain
     counts = [int(pool[c][0].shape[1]) for c, _, _ in draws]
     res = {"requests": nreq, "distinct_catalogues": npool,
+           "shells": {g: meas(g, 4, _lib.PREC_BF16X3, False).nb_bisp() for g in (128, 192, 256, 384)},
+           "triangles": {g: meas(g, 4, _lib.PREC_BF16X3, False).itot() for g in (128, 192, 256, 384)},
            "npart_min": min(counts), "npart_max": max(counts),
            "grids": {g: sum(1 for d in draws if d[1] == g) for g in (128, 192, 256, 384)},
            "orders": {o: sum(1 for d in draws if d[2] == o) for o in (2, 3, 4)},

@@ -68,9 +68,30 @@ class Context:
         return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=self.device)
 
     # ------------------------------------------------------------- assignment
+    def assign_workspace(self, grid, order, interlacing, npart):
+        d = AssignDesc(int(grid), int(order), int(bool(interlacing)), _lib.POS_BOX, 1.0,
+                       (ctypes.c_float * 3)(0.0, 0.0, 0.0))
+        n = lib().pbi4_assign_workspace(ctypes.byref(d), int(npart))
+        if n == 0:
+            raise Pbi4Error("pbi4_assign_workspace: invalid assignment descriptor")
+        return n
+
+    def density_workspace(self, grid, order, interlacing, box, npart):
+        d = DensityDesc(int(grid), int(order), int(bool(interlacing)), float(box), float(npart))
+        return lib().pbi4_density_workspace(self.h, ctypes.byref(d))
+
+    def _ws_or(self, ws, nbytes, name):
+        if ws is None:
+            return self._ws(nbytes)
+        _check_dev(ws, name, torch.uint8)
+        if ws.numel() < nbytes:
+            raise Pbi4Error(f"{name}: {ws.numel()} bytes, {nbytes} needed")
+        return ws
+
     def assign(self, pos, grid, order, interlacing=True, vel=None, box=1.0, rsd=None,
-               normalised=True, out=None, stream=None):
-        """Grids (1 or 2, N, N, N) float32 of the summed weights (powerI4.f90:68-249)."""
+               normalised=True, out=None, ws=None, stream=None):
+        """Grids (1 or 2, N, N, N) float32 of the summed weights (powerI4.f90:68-249).
+        ws: optional caller-owned uint8 device workspace (else one is allocated)."""
         _check_dev(pos, "pos", torch.float32)
         if pos.dim() != 2 or pos.shape[0] != 3:
             raise Pbi4Error("pos must have shape (3, npart)")
@@ -85,19 +106,19 @@ class Context:
         ng = 2 if interlacing else 1
         if out is None:
             out = torch.empty((ng, grid, grid, grid), dtype=torch.float32, device=self.device)
-        ws = self._ws(lib().pbi4_assign_workspace(ctypes.byref(d), npart))
+        ws = self._ws_or(ws, lib().pbi4_assign_workspace(ctypes.byref(d), npart), "ws")
         self._check(lib().pbi4_assign(self.h, ctypes.byref(d), _ptr(pos), _ptr(vel), npart,
                                       _ptr(out), _ptr(ws), ws.numel(), _stream(stream)))
         return out
 
     # ------------------------------------------------------------- density
-    def density(self, grids, grid, order, interlacing, box, npart, out=None, stream=None):
+    def density(self, grids, grid, order, interlacing, box, npart, out=None, ws=None, stream=None):
         """Half spectrum (N, N, N/2+1) complex64 of compute_density + fcomb."""
         _check_dev(grids, "grids", torch.float32)
         d = DensityDesc(int(grid), int(order), int(bool(interlacing)), float(box), float(npart))
         if out is None:
             out = torch.empty((grid, grid, grid // 2 + 1), dtype=torch.complex64, device=self.device)
-        ws = self._ws(lib().pbi4_density_workspace(self.h, ctypes.byref(d)))
+        ws = self._ws_or(ws, lib().pbi4_density_workspace(self.h, ctypes.byref(d)), "ws")
         self._check(lib().pbi4_compute_density(self.h, ctypes.byref(d), _ptr(grids), _ptr(out),
                                                _ptr(ws), ws.numel(), _stream(stream)))
         return out
@@ -277,8 +298,7 @@ class Context:
             pout = torch.empty((pk_desc.nbins, 5), dtype=torch.float64, device=self.device)
         if bout is None:
             bout = torch.empty((bk_desc.itot, rows), dtype=torch.float64, device=self.device)
-        if ws is None:
-            ws = self._ws(self.pk_bk_workspace(pk_desc, bk_desc))
+        ws = self._ws_or(ws, self.pk_bk_workspace(pk_desc, bk_desc), "ws")
         self._check(lib().pbi4_measure_pk_bk(self.h, ctypes.byref(pk_desc), ctypes.byref(bk_desc),
                                              _ptr(counts), _ptr(delta), _ptr(pout), _ptr(bout),
                                              _ptr(ws), ws.numel(), _stream(stream)))

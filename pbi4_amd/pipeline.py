@@ -86,37 +86,68 @@ class CataloguePlan:
         self.maps_grid = ctx.maps_grid(next(iter(self.desc.values())))
         self.bk_ws_bytes = max(ctx.pk_bk_workspace(self.pkdesc[f], self.desc[f]) for f in m.fields)
         self.rsd_coef = rsd_factor(m.z, m.om, m.ol)
+        self._layouts = {}            # npart -> arena_layout
+
+    def arena_layout(self, npart):
+        """Byte offsets of one catalogue's device buffers inside one workspace: the assignment
+        grids, the half-spectrum delta and one scratch area that the assignment, the density and
+        the fused P_l + B pass use in turn (they run in stream order).  Returns
+        (grids_off, delta_off, scratch_off, total)."""
+        hit = self._layouts.get(int(npart))
+        if hit is not None:
+            return hit
+        m, ctx = self.m, self.ctx
+        N = m.grid
+        ng = 2 if m.interlacing else 1
+        al = lambda x: (int(x) + 255) // 256 * 256
+        scratch = max(ctx.assign_workspace(N, m.order, m.interlacing, npart),
+                      ctx.density_workspace(N, m.order, m.interlacing, m.box, max(int(npart), 1)),
+                      self.bk_ws_bytes)
+        g_off = 0
+        d_off = al(g_off + 4 * ng * N ** 3)
+        s_off = al(d_off + 8 * N * N * (N // 2 + 1))
+        out = self._layouts[int(npart)] = (g_off, d_off, s_off, al(s_off + scratch))
+        return out
 
     def bytes_per_catalogue(self, npart):
-        """Device memory one catalogue of npart particles needs while in flight (workspace
-        estimate used for admission)."""
-        N = self.m.grid
-        V = N ** 3
-        h = N * N * (N // 2 + 1)
-        assign_ws = 12 * npart + 4 * 3 * (V // 64 + 1) + 8 * V * 2 * 2   # sort + regions (<= 2x)
-        grids = 8 * V
-        dens = 2 * 8 * h + 8 * h
-        return int(assign_ws + grids + dens + self.bk_ws_bytes + 8 * h) + (64 << 20)
+        """Device memory one catalogue of npart particles needs while in flight (its workspace
+        plus the small outputs)."""
+        return self.arena_layout(npart)[3] + (16 << 20)
 
-    def run(self, pos, vel, npart, stream=None, precision=None):
+    def run(self, pos, vel, npart, stream=None, precision=None, arena=None):
         """Enqueue every field of one catalogue; returns {field: (pout, bout)} device tensors.
         pos/vel: (3, npart) float32 box-unit positions and velocities (vel may be None when no
-        field is in redshift space)."""
+        field is in redshift space).  arena: optional uint8 device tensor of at least
+        arena_layout(npart)[3] bytes holding every intermediate buffer (else they are allocated);
+        it must not be touched by other streams until this catalogue's work has finished."""
         m, ctx = self.m, self.ctx
+        N = m.grid
+        ng = 2 if m.interlacing else 1
         out = {}
         s = stream if stream is not None else torch.cuda.current_stream()
+        grids = delta = scratch = None
+        if arena is not None:
+            g_off, d_off, s_off, total = self.arena_layout(npart)
+            if arena.numel() < total:
+                raise ValueError(f"arena holds {arena.numel()} bytes, {total} needed")
+            grids = arena[g_off:g_off + 4 * ng * N ** 3].view(torch.float32).view(ng, N, N, N)
+            delta = arena[d_off:d_off + 8 * N * N * (N // 2 + 1)].view(torch.complex64)
+            delta = delta.view(N, N, N // 2 + 1)
+            scratch = arena[s_off:total]
         with torch.cuda.stream(s):
             for f in m.fields:
                 rsd = tuple(float(np.float32(self.rsd_coef * l)) for l in f.los) if f.rsd else None
-                grids = ctx.assign(pos, m.grid, m.order, m.interlacing, vel=vel if f.rsd else None,
-                                   box=m.box, rsd=rsd, normalised=False, stream=s)
-                delta = ctx.density(grids, m.grid, m.order, m.interlacing, m.box, npart, stream=s)
-                del grids
+                g = ctx.assign(pos, N, m.order, m.interlacing, vel=vel if f.rsd else None,
+                               box=m.box, rsd=rsd, normalised=False, out=grids, ws=scratch,
+                               stream=s)
+                d = ctx.density(g, N, m.order, m.interlacing, m.box, npart, out=delta,
+                                ws=scratch, stream=s)
+                del g
                 desc = self.desc[f]
                 desc.npart = int(npart)
                 if precision is not None:
                     desc.precision = int(precision)
-                pout, bout = ctx.pk_bk(delta, self.counts, self.pkdesc[f], desc, stream=s)
+                pout, bout = ctx.pk_bk(d, self.counts, self.pkdesc[f], desc, ws=scratch, stream=s)
                 desc.precision = int(m.precision)
                 out[f] = (pout, bout)
         return out

@@ -1,11 +1,15 @@
 """Device-resident serving loop for covariance-style workloads (continuous batching).
 
-Requests (one catalogue each, any grid / order / particle count) queue in arrival order.  A
-request is admitted when a slot is free and the device has room for its in-flight memory
-(estimated per request, checked against hipMemGetInfo plus what torch's caching allocator holds
-unused); each admitted request runs its whole pipeline on its slot's own HIP stream, so requests
-that arrive while others are on the GPU start as soon as a slot and memory free up and their
-kernels overlap on the device.  Plans (rocFFT plans, exact triangle counts, bin and shell
+Requests (one catalogue each, any grid / order / particle count) queue in arrival order.  Each
+slot owns one device workspace (arena) that holds every intermediate buffer of the catalogue it
+serves (assignment grids, half spectrum, sort / FFT / shell-map / GEMM scratch); arenas are kept
+between requests and grow only when a request needs more, so a stream of requests does not
+allocate and free tens of GB per catalogue (with a caching allocator that fragments, and frees
+synchronise the device).  A request is admitted when a slot is free and its arena is large
+enough, or can grow within the free device memory (hipMemGetInfo plus what torch's caching
+allocator holds unused, plus the slot's own arena, released first); each admitted request runs
+its whole pipeline on its slot's own HIP stream, so requests that arrive while others are on the
+GPU start as soon as a slot and memory free up and their kernels overlap on the device.  Plans (rocFFT plans, exact triangle counts, bin and shell
 tables) are built once per request shape and stay resident.  Completion is detected with HIP
 events; results come back through pinned host buffers.  Latency = completion - arrival.
 
@@ -65,6 +69,11 @@ class _Slot:
     event: torch.cuda.Event = None
     outputs: dict = None
     host: dict = field(default_factory=dict)
+    arena: torch.Tensor = None
+
+    @property
+    def capacity(self):
+        return 0 if self.arena is None else int(self.arena.numel())
 
 
 class ServingEngine:
@@ -75,16 +84,24 @@ class ServingEngine:
         self.finished = []
         self.plans = {}
         self.reserve = int(reserve_bytes)
-        self.free_bytes_fn = free_bytes_fn or self._free_bytes
+        self._free_fn = free_bytes_fn          # None: the device's (no bound-method cycle)
         self.t0 = time.perf_counter()
         self.deferred = 0
 
     # ---------------------------------------------------------------- memory / plans
-    def _free_bytes(self):
+    def free_bytes(self):
+        if self._free_fn is not None:
+            return self._free_fn()
         free, _ = torch.cuda.mem_get_info(self.ctx.device)
         stats = torch.cuda.memory_stats(self.ctx.device)
         cached = stats.get("reserved_bytes.all.current", 0) - stats.get("allocated_bytes.all.current", 0)
         return free + max(0, cached)
+
+    def release(self):
+        """Drop the idle slots' workspaces (they return to torch's caching allocator)."""
+        for s in self.slots:
+            if s.req is None:
+                s.arena = None
 
     def plan(self, m):
         p = self.plans.get(m)
@@ -119,19 +136,44 @@ class ServingEngine:
         self.finished.extend(done)
         while self.queue:
             r = self.queue[0]
-            slot = next((s for s in self.slots if s.req is None), None)
             plan = self.plan(r.measurement)
-            need = plan.bytes_per_catalogue(r.npart) + self.reserve
-            d = admission(self.now(), r.arrival, slot is not None, need,
-                          self.free_bytes_fn() if slot is not None else 0, self.in_flight())
+            size = plan.arena_layout(r.npart)[3]
+            free_slots = [s for s in self.slots if s.req is None]
+            # a free slot whose arena already fits (the smallest such), else the free slot with
+            # the largest arena, which grows
+            fits = [s for s in free_slots if s.capacity >= size]
+            slot = (min(fits, key=lambda s: s.capacity) if fits else
+                    max(free_slots, key=lambda s: s.capacity) if free_slots else None)
+            grow = slot is not None and slot.capacity < size
+            need = (size + self.reserve) if grow else 0
+            free = (self.free_bytes() + slot.capacity) if grow else 0
+            d = admission(self.now(), r.arrival, slot is not None, need, free, self.in_flight())
             if d == REJECT:
                 raise MemoryError(f"request {r.rid} needs {need} bytes, more than the device has")
+            if d == ADMIT and grow and not self._grow(slot, size):
+                d = DEFER if self.in_flight() > 0 else REJECT
+                if d == REJECT:
+                    raise MemoryError(f"request {r.rid}: a {size}-byte workspace does not fit")
             if d != ADMIT:
                 self.deferred += d == DEFER and slot is not None
                 break
             self.queue.popleft()
             self._launch(slot, r, plan)
         return done
+
+    def _grow(self, slot, size):
+        """Replace the (idle) slot's arena by one of `size` bytes; False if the device cannot
+        hold it now."""
+        slot.arena = None
+        try:
+            slot.arena = torch.empty(size, dtype=torch.uint8, device=self.ctx.device)
+        except torch.OutOfMemoryError:
+            torch.cuda.empty_cache()
+            try:
+                slot.arena = torch.empty(size, dtype=torch.uint8, device=self.ctx.device)
+            except torch.OutOfMemoryError:
+                return False
+        return True
 
     def _launch(self, slot, r, plan):
         r.admitted = self.now()
@@ -141,7 +183,7 @@ class ServingEngine:
         r.pos.record_stream(slot.stream)
         if r.vel is not None:
             r.vel.record_stream(slot.stream)
-        slot.outputs = plan.run(r.pos, r.vel, r.npart, stream=slot.stream)
+        slot.outputs = plan.run(r.pos, r.vel, r.npart, stream=slot.stream, arena=slot.arena)
         with torch.cuda.stream(slot.stream):
             for f, (pout, bout) in slot.outputs.items():
                 hp = torch.empty(pout.shape, dtype=pout.dtype, pin_memory=True)
