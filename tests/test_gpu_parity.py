@@ -332,9 +332,10 @@ def test_triangle_sums(ctx, prec, tol, nb, nell, K):
 
 def _bk_close(mine, ref, rows, rtol=1e-4):
     """Rows 0-5, 7, 8: k's, pow, BSN, Ntr; rows 6, 9, 10: B.  B0 is compared relative to
-    max(|B0_ref|, Gaussian scale sqrt(P1 P2 P3 / Ntr)), the natural size of B in that bin;
-    B2 and B4 relative to max(|B_l|, |B0|, Gaussian scale), as P2 and P4 are relative to P0
-    (B_l crosses zero where B0 does not)."""
+    max(|B0_ref|, sqrt(P1 P2 P3 / Ntr)); B2 and B4 relative to max(|B_l|, |B0|, that scale), as
+    P2 and P4 are relative to P0 (B_l crosses zero where B0 does not).  The Gaussian standard
+    deviation of B in a bin is sqrt(s_123 P1 P2 P3 / (Ntr kf^3)); the floor used here leaves out
+    the 1/kf^3 (kf = 2 pi / L < 1), so it is the stricter of the two."""
     for r in (0, 1, 2, 8):
         assert np.array_equal(mine[r], ref[r]), r
     for r in (3, 4, 5, 7):
