@@ -380,6 +380,12 @@ GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precis
     PBI4_CHECK(K >= 4 && K % 4 == 0 && Kstride % 4 == 0, PBI4_EINVAL,
                "maps length must be a positive multiple of 4");
     g.rf = g.npair >= 900 ? 4 : g.npair >= 300 ? 2 : 1;
+    // 12 column tiles with 4 row tiles hold 192 accumulator registers: one wave per SIMD; two
+    // row tiles keep two (192-column shapes: 161 -> 144 ms at 64 shells on 384^3)
+    if (g.cf >= 12 && g.rf > 2) g.rf = 2;
+    // few pairs and several column tiles: one row block reads the maps once instead of once per
+    // block (16 shells x 3 multipoles on 100^3: 98 -> 89 us; 21 shells on 128^3: 336 -> 242 us)
+    if (g.npair <= 256 && g.cf >= 3) g.rf = 4;
     g.nrowblocks = (int)ceil_div(g.npair, 64 * g.rf);
     // ~2048 work-groups, K chunks a multiple of the LDS stage
     const int kt = g.cf <= 6 ? 64 : 32;
