@@ -36,8 +36,8 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--pool", type=int, default=4, help="distinct resident catalogues")
     ap.add_argument("--lattice", type=int, default=256)

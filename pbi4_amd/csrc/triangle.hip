@@ -387,10 +387,13 @@ GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precis
     // block (16 shells x 3 multipoles on 100^3: 98 -> 89 us; 21 shells on 128^3: 336 -> 242 us)
     if (g.npair <= 256 && g.cf >= 3) g.rf = 4;
     g.nrowblocks = (int)ceil_div(g.npair, 64 * g.rf);
-    // ~2048 work-groups, K chunks a multiple of the LDS stage
+    // ~2048 work-groups, K chunks a multiple of the LDS stage and at least 2048 long: each chunk
+    // writes a partial tile that the FP64 reduce reads back, so short chunks on small problems
+    // cost more in partials than they gain in parallelism (16 shells x 3 on 100^3: GEMM + reduce
+    // 197 -> 118 us)
     const int kt = g.cf <= 6 ? 64 : 32;
     int64_t target = std::max<int64_t>(kXcds, 2048 / g.nrowblocks);
-    int64_t kc = std::max<int64_t>(kt * 4, align_up(ceil_div(K, target), kt));
+    int64_t kc = std::max<int64_t>(2048, align_up(ceil_div(K, target), kt));
     g.kchunk = (int)kc;
     g.nkchunks = (int)ceil_div(K, kc);
     g.nkchunks_pad = (int)align_up(g.nkchunks, kXcds);
