@@ -69,9 +69,9 @@ __device__ __forceinline__ void grid_coord(const PosXform &x, float p, int &c, f
     c = ci;
 }
 
-// One particle's raw components.  Loads are unconditional (the caller clamps the index into
-// range; a component without RSD reads its position twice) so that the loads of several
-// particles issue as one batch; guarded loads become branches with a wait after each.
+// One particle's raw components.  Loads are not guarded per thread (the caller clamps the index
+// into range) so that the loads of several particles issue as one batch; a guarded load becomes
+// a branch with a wait after it.
 struct RawParticle {
     float p[3], v[3];
 };
@@ -80,10 +80,13 @@ __device__ __forceinline__ RawParticle load_particle(const PosXform &x, const fl
                                                      const float *vel, int64_t i, int64_t npart) {
     RawParticle r;
 #pragma unroll
+    for (int d = 0; d < 3; ++d) r.p[d] = pos[(int64_t)d * npart + i];
+    // the condition is the same for every thread (kernel arguments): a scalar branch, no
+    // divergence, and no second load of the position for components without a shift
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const float *vd = (x.has_vel && x.rsd[d] != 0.0f) ? vel : pos;
-        r.p[d] = pos[(int64_t)d * npart + i];
-        r.v[d] = vd[(int64_t)d * npart + i];
+        if (x.has_vel && x.rsd[d] != 0.0f) r.v[d] = vel[(int64_t)d * npart + i];
+        else r.v[d] = 0.0f;
     }
     return r;
 }
