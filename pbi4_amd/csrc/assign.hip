@@ -287,7 +287,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const uint32_t *counts, int6
     if (threadIdx.x == 1023) offsets[ntiles] = part[1023];
 }
 
-constexpr int kScatterPer = 8;   // particles per thread per LDS round
+constexpr int kScatterPer = 4;   // particles per thread per LDS round
 
 __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T, int NT,
                                                                int64_t ntiles, const float *pos,
