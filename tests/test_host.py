@@ -173,3 +173,27 @@ def test_bench_finds_the_roofline_kernels_traffic():
     spec.loader.exec_module(b)
     t = b.pmc_traffic("k_deposit<4, 2, 20>")
     assert t is not None and t > 335544320          # at least the algorithmic bytes per launch
+
+
+def test_assign_tile_edge_follows_particle_density():
+    # the assignment workspace holds the tile regions, (T + 4)^3 cells per grid and tile: the
+    # tile edge T is the largest divisor of N in [4, 16] with at most 8192 particles per tile on
+    # average, but not below 8 while a divisor in [8, 16] exists (no GPU needed for the query)
+    import ctypes
+    from pbi4_amd import _lib
+    L = _lib.lib()
+
+    def ws(N, npart):
+        d = _lib.AssignDesc(N, 4, 1, 1, 1000.0, (ctypes.c_float * 3)(0, 0, 0))
+        return L.pbi4_assign_workspace(ctypes.byref(d), npart)
+
+    def regions(N, T):
+        return (N // T) ** 3 * 2 * (T + 4) ** 3 * 4
+
+    def rest(N, npart):                 # sort arrays and tile counters, below the regions
+        return ws(N, npart) - 12 * npart
+
+    for N, npart, T in [(128, 1_000_000, 16), (128, 1 << 24, 8), (256, 1 << 24, 16),
+                        (192, 1 << 24, 12), (92, 60000, 4), (20, 1 << 20, 10)]:
+        r = rest(N, npart)
+        assert regions(N, T) <= r <= regions(N, T) + 3 * 4 * ((N // T) ** 3 + 1) + 8192, (N, npart)
