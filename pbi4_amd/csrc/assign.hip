@@ -617,7 +617,7 @@ __device__ __forceinline__ void deposit_chunk(Acc *acc, int R_, int GS, int per,
         q[m][1] = ok ? sy[p] : 0u;
         q[m][2] = ok ? sz[p] : 0u;
     }
-    __syncthreads();   // the accumulators are cleared before the first chunk of a round
+    __syncthreads();   // the accumulators are cleared before the first chunk of a tile
     // orders 3 and 4: the four lanes of a particle are a quad; lane ix evaluates the weight of
     // stencil point ix along each axis and the quad shares them by DPP broadcasts
     const WeightPoly wp = weight_poly<ORDER>(ix);
@@ -896,16 +896,6 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
 }
 
 }  // namespace
-
-// Work-groups of kSortThreads threads with `lds` bytes of dynamic LDS that the device holds at
-// once (all compute units).
-static int resident_wgs(const void *kern, size_t lds) {
-    int dev = 0, cus = 0, per_cu = 0;
-    PBI4_HIP(hipGetDevice(&dev));
-    PBI4_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    PBI4_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kSortThreads, lds));
-    return std::max(1, cus) * std::max(1, per_cu);
-}
 
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
