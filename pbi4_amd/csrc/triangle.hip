@@ -244,7 +244,16 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const float *partials, int 
     const int per = (nchunks + 7) / 8;
     const int k0 = part * per, k1 = min(nchunks, k0 + per);
     double s = 0.0;
-    for (int k = k0; k < k1; ++k) s += (double)partials[((int64_t)k * npair_pad + p) * ncol_pad + c];
+    // loads issued 8 at a time, added in chunk order (the sum stays in a fixed order)
+    int k = k0;
+    for (; k + 8 <= k1; k += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partials[((int64_t)(k + u) * npair_pad + p) * ncol_pad + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += (double)v[u];
+    }
+    for (; k < k1; ++k) s += (double)partials[((int64_t)k * npair_pad + p) * ncol_pad + c];
     red[part][lo] = s;
     __syncthreads();
     if (part == 0 && ok) {
