@@ -25,6 +25,12 @@ namespace pbi4 {
 
 namespace {
 
+// A sorted particle: per axis 4 bits of cell within the tile and 28 bits of fraction, stored
+// as one 12-byte record (the scatter writes it with one store, the deposit loads it with one).
+struct Pk3 {
+    uint32_t c[3];
+};
+
 constexpr int kHalo = 2;          // region cells below the tile origin
 constexpr int kMaxR = 16 + 4;     // T <= 16
 // LDS plane stride of the deposit is R^2 + kPadZ words: with R^2 = 400 a z step of 4 cells is a
@@ -292,8 +298,7 @@ constexpr int kScatterPer = 4;   // particles per thread per LDS round
 __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T, int NT,
                                                                int64_t ntiles, const float *pos,
                                                                const float *vel, int64_t npart,
-                                                               uint32_t *cursor, uint32_t *sx,
-                                                               uint32_t *sy, uint32_t *sz,
+                                                               uint32_t *cursor, Pk3 *spk,
                                                                int hashed) {
     // LDS: the round's counters (one per tile, or the hashed slots followed by their keys), then
     // the list of counters this round touched and its length; only those are flushed and cleared
@@ -364,9 +369,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T
             if (ctr[k] == -1) continue;
             const uint32_t slot = ctr[k] >= 0 ? lhist[ctr[k]] + rank[k] : rank[k];
             if (slot >= (uint32_t)npart) continue;   // cannot happen; keeps a bad rank in bounds
-            sx[slot] = pk[k][0];
-            sy[slot] = pk[k][1];
-            sz[slot] = pk[k][2];
+            spk[slot] = Pk3{{pk[k][0], pk[k][1], pk[k][2]}};
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) {
@@ -600,8 +603,7 @@ struct Slab {
 template <int ORDER, int NG, int RC, bool SLAB, typename Acc>
 __device__ __forceinline__ void deposit_chunk(Acc *acc, int R_, int GS, int per, float fix,
                                               Slab sl, uint32_t c0, uint32_t cend,
-                                              const uint32_t *sx, const uint32_t *sy,
-                                              const uint32_t *sz) {
+                                              const Pk3 *spk) {
     constexpr int single = NG == 1;
     constexpr int X = DepositLanes<ORDER>::X;
     const int R = RC ? RC : R_;
@@ -613,9 +615,11 @@ __device__ __forceinline__ void deposit_chunk(Acc *acc, int R_, int GS, int per,
     for (int m = 0; m < kDepositPerMax; ++m) {
         const uint32_t p = cb + (uint32_t)(lane + 64 * m);
         const bool ok = m < per && p < cend;
-        q[m][0] = ok ? sx[p] : 0u;
-        q[m][1] = ok ? sy[p] : 0u;
-        q[m][2] = ok ? sz[p] : 0u;
+        Pk3 v{{0u, 0u, 0u}};
+        if (ok) v = spk[p];
+        q[m][0] = v.c[0];
+        q[m][1] = v.c[1];
+        q[m][2] = v.c[2];
     }
     __syncthreads();   // the accumulators are cleared before the first chunk of a tile
     // orders 3 and 4: the four lanes of a particle are a quad; lane ix evaluates the weight of
@@ -753,9 +757,7 @@ __device__ __forceinline__ int block_max(int v, int *red) {
 template <int ORDER, int NG, int RC>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
                                                              const uint32_t *offsets,
-                                                             const uint32_t *sx,
-                                                             const uint32_t *sy,
-                                                             const uint32_t *sz,
+                                                             const Pk3 *spk,
                                                              float *regions) {
     // 32-bit cells of NG grids, or 64-bit cells of one slab: NG kMaxR (kMaxR^2 + kPadZ) / 2 of
     // them, a whole grid for NG = 2 and ceil(R / 2) planes for NG = 1
@@ -785,7 +787,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
         for (int i = threadIdx.x; i < T3; i += blockDim.x) acc[i] = 0;
         __syncthreads();
         for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x)
-            atomicAdd(&acc[((sz[i] >> 28) * T + (sy[i] >> 28)) * T + (sx[i] >> 28)], 1);
+            atomicAdd(&acc[((spk[i].c[2] >> 28) * T + (spk[i].c[1] >> 28)) * T + (spk[i].c[0] >> 28)], 1);
         __syncthreads();
         int best = 0;
         if ((int)threadIdx.x < T * T) {                     // one (x, y) column per thread
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
         for (int i = threadIdx.x; i < NG * GS; i += blockDim.x) acc[i] = 0;
         for (uint32_t c0 = b; c0 < e; c0 += chunk)
             deposit_chunk<ORDER, NG, RC, false>(acc, R, GS, per, fix, Slab{0, 0, R}, c0,
-                                                min(e, c0 + chunk), sx, sy, sz);
+                                                min(e, c0 + chunk), spk);
         __syncthreads();
         if (sgn) flush_region(acc, T, R, PZ, GS, ncell, 1.0f / fix, out);
         else flush_region(reinterpret_cast<unsigned *>(acc), T, R, PZ, GS, ncell, 1.0f / fix, out);
@@ -821,7 +823,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(int T, int R_,
         for (int i = threadIdx.x; i < nsl; i += blockDim.x) acc64[i] = 0;
         for (uint32_t c0 = b; c0 < e; c0 += chunk)
             deposit_chunk<ORDER, NG, RC, true>(acc64, R, GS, per, fix, sl, c0, min(e, c0 + chunk),
-                                               sx, sy, sz);
+                                               spk);
         __syncthreads();
         for (int i = threadIdx.x; i < (sl.zhi - sl.zlo) * R * R; i += blockDim.x) {
             const int c = i / (R * R), rem = i - c * (R * R), bb = rem / R, aa = rem - bb * R;
@@ -944,9 +946,7 @@ size_t assign_workspace(const AssignPlan &p, int64_t npart) {
     a.take<uint32_t>(p.ntiles + 1);     // counts
     a.take<uint32_t>(p.ntiles + 1);     // offsets
     a.take<uint32_t>(p.ntiles + 1);     // cursor
-    a.take<uint32_t>(npart);            // sorted x
-    a.take<uint32_t>(npart);            // sorted y
-    a.take<uint32_t>(npart);            // sorted z
+    a.take<Pk3>(npart);                 // sorted packed coordinates
     a.take<float>((size_t)p.ntiles * p.ngrids * p.R * p.R * p.R);   // regions
     return a.used + 256;
 }
@@ -959,9 +959,7 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
     uint32_t *counts = ws.take<uint32_t>(p.ntiles + 1);
     uint32_t *offsets = ws.take<uint32_t>(p.ntiles + 1);
     uint32_t *cursor = ws.take<uint32_t>(p.ntiles + 1);
-    uint32_t *sx = ws.take<uint32_t>(npart);
-    uint32_t *sy = ws.take<uint32_t>(npart);
-    uint32_t *sz = ws.take<uint32_t>(npart);
+    Pk3 *spk = ws.take<Pk3>(npart);
     const int R3 = p.R * p.R * p.R;
     float *regions = ws.take<float>((size_t)p.ntiles * p.ngrids * R3);
 
@@ -987,13 +985,13 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
         hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, p.ntiles, offsets, cursor);
         if (npart > 0)
             hipLaunchKernelGGL(k_tile_scatter, dim3(nwg_s), dim3(kSortThreads), lds_scatter, s, x, p.T,
-                               p.NT, p.ntiles, pos, vel, npart, cursor, sx, sy, sz, hashed);
+                               p.NT, p.ntiles, pos, vel, npart, cursor, spk, hashed);
     }
     {
         TimedScope ts(timer, "pbi4_deposit", s);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3((unsigned)p.ntiles), dim3(kDepositThreads), 0, s, p.T,
-                               p.R, offsets, sx, sy, sz, regions);
+                               p.R, offsets, spk, regions);
         };
 #define PBI4_DEPOSIT(O, G, RC) launch(k_deposit<O, G, RC>)
         const bool two = p.ngrids == 2;
