@@ -162,6 +162,15 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
     size_t len[3] = {(size_t)N, (size_t)N, (size_t)N};
     rocfft_plan_description desc = nullptr;
     fft_check(rocfft_plan_description_create(&desc), "rocfft_plan_description_create");
+    if (kind == 2) {
+        // forward c2c of one N^3 field given as planar parts (real part, imaginary part)
+        size_t strides[3] = {1, (size_t)N, (size_t)N * N};
+        const size_t dist = (size_t)N * N * N;
+        fft_check(rocfft_plan_description_set_data_layout(
+                      desc, rocfft_array_type_complex_planar, rocfft_array_type_complex_interleaved,
+                      nullptr, nullptr, 3, strides, dist, 3, strides, dist),
+                  "rocfft_plan_description_set_data_layout");
+    }
     if (kind == 1) {
         const size_t h = N / 2 + 1;
         size_t in_strides[3] = {1, h, h * N};
@@ -172,8 +181,9 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
                   "rocfft_plan_description_set_data_layout");
     }
     fft_check(rocfft_plan_create(&p.plan, rocfft_placement_notinplace,
-                                 kind == 0 ? rocfft_transform_type_real_forward
-                                           : rocfft_transform_type_real_inverse,
+                                 kind == 0   ? rocfft_transform_type_real_forward
+                                 : kind == 2 ? rocfft_transform_type_complex_forward
+                                             : rocfft_transform_type_real_inverse,
                                  rocfft_precision_single, 3, len, (size_t)batch, desc),
               "rocfft_plan_create");
     rocfft_plan_description_destroy(desc);
@@ -181,13 +191,13 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
     return ctx->plans[key] = p;
 }
 
-void fft_exec(FftPlan &p, void *in, void *out, void *work, hipStream_t s) {
+void fft_exec(FftPlan &p, void *in, void *out, void *work, hipStream_t s, void *in2 = nullptr) {
     rocfft_execution_info info = nullptr;
     fft_check(rocfft_execution_info_create(&info), "rocfft_execution_info_create");
     fft_check(rocfft_execution_info_set_stream(info, s), "rocfft_execution_info_set_stream");
     if (p.work) fft_check(rocfft_execution_info_set_work_buffer(info, work, p.work),
                           "rocfft_execution_info_set_work_buffer");
-    void *ib[1] = {in}, *ob[1] = {out};
+    void *ib[2] = {in, in2}, *ob[1] = {out};
     const rocfft_status st = rocfft_execute(p.plan, ib, ob, info);
     rocfft_execution_info_destroy(info);
     fft_check(st, "rocfft_execute");
@@ -329,11 +339,13 @@ struct DensityWs {
     float2 *G;
     void *fftwork;
 };
+// Interlaced: one complex FFT of grid1 + i grid2 (the array the reference transforms,
+// pbi4.py:337), the two grids passed as its planar real and imaginary parts; single grid: r2c.
 DensityWs density_layout(pbi4_ctx *ctx, const pbi4_density_desc &d, Arena &a) {
-    const int ng = d.interlacing ? 2 : 1;
-    FftPlan &p = get_plan(ctx, 0, d.grid, ng, 0);
+    const int64_t N = d.grid;
+    FftPlan &p = d.interlacing ? get_plan(ctx, 2, d.grid, 1, 0) : get_plan(ctx, 0, d.grid, 1, 0);
     DensityWs w;
-    w.G = a.take<float2>((size_t)ng * d.grid * d.grid * (d.grid / 2 + 1));
+    w.G = a.take<float2>(d.interlacing ? (size_t)(N * N * N) : (size_t)(N * N * (N / 2 + 1)));
     w.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
     return w;
 }
@@ -578,18 +590,18 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
         hipStream_t s = (hipStream_t)stream;
         Arena a(ws, ws_bytes);
         DensityWs w = density_layout(ctx, *d, a);
-        const int ng = d->interlacing ? 2 : 1;
-        FftPlan &p = get_plan(ctx, 0, d->grid, ng, 0);
+        const int64_t V = (int64_t)d->grid * d->grid * d->grid;
+        FftPlan &p = d->interlacing ? get_plan(ctx, 2, d->grid, 1, 0) : get_plan(ctx, 0, d->grid, 1, 0);
         {
             TimedScope ts(&ctx->timer, "pbi4_r2c", s);
-            fft_exec(p, const_cast<float *>(grids), w.G, w.fftwork, s);
+            fft_exec(p, const_cast<float *>(grids), w.G, w.fftwork, s,
+                     d->interlacing ? const_cast<float *>(grids) + V : nullptr);
         }
         const double kF3 = kf3_fcomb(d->box);
         const double cf = (d->interlacing ? 1.0 / (4.0 * kF3) : 1.0 / kF3) / d->npart;
-        const int64_t half = (int64_t)d->grid * d->grid * (d->grid / 2 + 1);
         TimedScope ts(&ctx->timer, "pbi4_combine", s);
         launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order), (float)cf, w.G,
-                       w.G + half, static_cast<float2 *>(delta), s);
+                       nullptr, static_cast<float2 *>(delta), s);
     });
 }
 
