@@ -77,8 +77,11 @@ struct ShellInfo {
     int N, nb, max_d2, R;
     std::vector<int16_t> table;   // d2 -> shell (0 = none)
     std::vector<double> nmodes;   // full-grid modes per shell
+    std::vector<int32_t> d2range;  // per shell: smallest, largest d2 (empty shell: 1, 0)
+    bool contiguous = false;       // every d2 inside a shell's range belongs to that shell
     int16_t *table_d = nullptr;
     double *nmodes_d = nullptr;
+    int32_t *d2range_d = nullptr;
 };
 
 struct PkTable {
@@ -105,6 +108,7 @@ struct pbi4_ctx {
     std::map<std::tuple<int, float, float, float, int>, PkTable> pk_tables;
     std::map<std::tuple<int, int, float, float>, std::unique_ptr<ShellInfo>> shells;
     std::map<std::tuple<int, int>, float *> windows;   // (N, order) -> device sinc^p table
+    std::map<int, float2 *> twiddles;                  // N' -> device e^{2 pi i k / N'}, k < N'
     std::map<std::tuple<int, int, int>, FftPlan> plans; // (kind, N, batch/ext)
     std::map<std::tuple<int, int, float, float>, CountsCache> counts;
     std::map<std::tuple<int, int, float, float, int, int>, int32_t *> tri_tables;
@@ -153,6 +157,8 @@ struct RocfftInit {
 
 // kind 0: r2c of N^3 with `batch` grids (default layouts).
 // kind 1: c2r of N^3 with `batch` maps, real output per-map distance `ext` floats.
+// kind 2: forward c2c of one N^3 field from planar real / imaginary parts.
+// kind 3: 2D c2r of `batch` N^2 planes (the maps after their z transform).
 FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
     static RocfftInit init;
     auto key = std::make_tuple(kind * 1000000 + batch, N, (int)ext);
@@ -171,6 +177,15 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
                       nullptr, nullptr, 3, strides, dist, 3, strides, dist),
                   "rocfft_plan_description_set_data_layout");
     }
+    if (kind == 3) {
+        // c2r over (y, x) of `batch` N x N planes: half spectra [y][x'] in, real [y][x] out
+        const size_t h = N / 2 + 1;
+        size_t in_strides[2] = {1, h}, out_strides[2] = {1, (size_t)N};
+        fft_check(rocfft_plan_description_set_data_layout(
+                      desc, rocfft_array_type_hermitian_interleaved, rocfft_array_type_real, nullptr,
+                      nullptr, 2, in_strides, h * N, 2, out_strides, (size_t)N * N),
+                  "rocfft_plan_description_set_data_layout");
+    }
     if (kind == 1) {
         const size_t h = N / 2 + 1;
         size_t in_strides[3] = {1, h, h * N};
@@ -184,7 +199,8 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
                                  kind == 0   ? rocfft_transform_type_real_forward
                                  : kind == 2 ? rocfft_transform_type_complex_forward
                                              : rocfft_transform_type_real_inverse,
-                                 rocfft_precision_single, 3, len, (size_t)batch, desc),
+                                 rocfft_precision_single, kind == 3 ? 2 : 3, len, (size_t)batch,
+                                 desc),
               "rocfft_plan_create");
     rocfft_plan_description_destroy(desc);
     fft_check(rocfft_plan_get_work_buffer_size(p.plan, &p.work), "rocfft_plan_get_work_buffer_size");
@@ -225,6 +241,44 @@ float *get_window(pbi4_ctx *ctx, int N, int order) {
         w[i] = (float)y;
     }
     return ctx->windows[key] = dev_upload(ctx, w);
+}
+
+// e^{2 pi i k / N'} for k < N', from FP64, with entry N'-k set to the exact conjugate of k.
+float2 *get_twiddle(pbi4_ctx *ctx, int Np) {
+    auto it = ctx->twiddles.find(Np);
+    if (it != ctx->twiddles.end()) return it->second;
+    std::vector<float2> t(Np);
+    for (int k = 0; k <= Np / 2; ++k) {
+        const double a = 2.0 * M_PI * (double)k / (double)Np;
+        t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+        if (k > 0) t[Np - k] = make_float2(t[k].x, -t[k].y);
+    }
+    return ctx->twiddles[Np] = dev_upload(ctx, t);
+}
+
+// The maps' inverse transform.  Band-limited maps grid (N' < N, so 3R < N'): the z transform is
+// fused into the shell filter (k_filter_z) and a batched 2D c2r over (y, x) finishes; one plan
+// over all maps when their stride is N'^3, else one execution per map.  Full grid: the shell
+// filter writes half spectra and a 3D c2r transforms them.
+bool maps_fused(int Np, int grid, const ShellInfo &sh) { return Np < grid && sh.contiguous; }
+
+FftPlan &maps_plan(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
+                   int64_t Kstride) {
+    if (!maps_fused(Np, grid, sh)) return get_plan(ctx, 1, Np, nmaps, Kstride);
+    const bool one = Kstride == (int64_t)Np * Np * Np;
+    return get_plan(ctx, 3, Np, one ? nmaps * Np : Np, 0);
+}
+
+void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, int64_t Kstride,
+              float2 *spectra, float *maps, void *work, hipStream_t s) {
+    FftPlan &p = maps_plan(ctx, sh, Np, grid, nmaps, Kstride);
+    TimedScope ts(&ctx->timer, "pbi4_c2r", s);
+    if (!maps_fused(Np, grid, sh) || Kstride == (int64_t)Np * Np * Np) {
+        fft_exec(p, spectra, maps, work, s);
+        return;
+    }
+    const size_t spec = (size_t)Np * Np * (Np / 2 + 1);
+    for (int m = 0; m < nmaps; ++m) fft_exec(p, spectra + m * spec, maps + m * Kstride, work, s);
 }
 
 // kF3 = twopi*twopi*twopi/box/box/box in REAL(4) (powerI4.f90:271)
@@ -285,8 +339,23 @@ ShellInfo &get_shells(pbi4_ctx *ctx, int N, int nb, float kbin, float kcenter) {
                 if (sh) s->nmodes[sh - 1] += mult[a] * mult[b] * mult[c];
             }
         }
+    s->d2range.resize(2 * nb);
+    for (int i = 0; i < nb; ++i) {
+        s->d2range[2 * i] = 1;
+        s->d2range[2 * i + 1] = 0;
+    }
+    for (int d2 = s->max_d2; d2 >= 0; --d2)
+        if (const int sh = s->table[d2]) {
+            s->d2range[2 * (sh - 1)] = d2;
+            if (s->d2range[2 * (sh - 1) + 1] < d2) s->d2range[2 * (sh - 1) + 1] = d2;
+        }
+    s->contiguous = true;
+    for (int i = 0; i < nb; ++i)
+        for (int d2 = s->d2range[2 * i]; d2 <= s->d2range[2 * i + 1]; ++d2)
+            if (s->table[d2] != i + 1) s->contiguous = false;
     s->table_d = dev_upload(ctx, s->table);
     s->nmodes_d = dev_upload(ctx, s->nmodes);
+    s->d2range_d = dev_upload(ctx, s->d2range);
     auto &ref = *s;
     ctx->shells[key] = std::move(s);
     return ref;
@@ -411,7 +480,7 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a, int nbins_pk 
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, L.nell, L.K, L.Kstride, d.precision);
     BinPlan b = make_bin_plan(d.grid, nbins_pk, d.nbins, 0, d.los, 1.0f);
-    FftPlan &p = get_plan(ctx, 1, L.Np, L.nmaps, L.Kstride);
+    FftPlan &p = maps_plan(ctx, sh, L.Np, d.grid, L.nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(d.nbins);
     L.scale_exp = a.take<float>(d.nbins);
@@ -436,20 +505,21 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
     for (int i = 0; i < 3; ++i) f.los[i] = (float)d->los[i];
     f.shell_table = sh.table_d;
     f.max_d2 = sh.max_d2;
+    f.shell_d2 = sh.d2range_d;
     f.f2 = static_cast<const float2 *>(delta2);
     f.f4 = static_cast<const float2 *>(delta4);
     PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
                "delta2/delta4 are only used with multipoles");
     {
         TimedScope ts(&ctx->timer, "pbi4_filter", s);
-        launch_filter(f, static_cast<const float2 *>(delta), L.shell_sum, sh.nmodes_d, L.spectra,
-                      L.scale_exp, s);
+        if (maps_fused(L.Np, d->grid, sh))
+            launch_filter_z(f, static_cast<const float2 *>(delta), L.shell_sum,
+                            get_twiddle(ctx, L.Np), L.spectra, L.scale_exp, s);
+        else
+            launch_filter(f, static_cast<const float2 *>(delta), L.shell_sum, sh.nmodes_d,
+                          L.spectra, L.scale_exp, s);
     }
-    FftPlan &p = get_plan(ctx, 1, L.Np, L.nmaps, L.Kstride);
-    {
-        TimedScope ts(&ctx->timer, "pbi4_c2r", s);
-        fft_exec(p, L.spectra, L.maps, L.fftwork, s);
-    }
+    maps_c2r(ctx, sh, L.Np, d->grid, L.nmaps, L.Kstride, L.spectra, L.maps, L.fftwork, s);
     launch_gemm(L.g, L.maps, L.gemm_partials, s, &ctx->timer);
     launch_gemm_reduce(L.g, L.gemm_partials, L.C, s);
     // kf3 = kf**3 in REAL(4) (bispectrumI4.f90:46); powSN = 1/dble(npart)/kf3 (:47)
@@ -494,7 +564,7 @@ BkCrossLayout bk_cross_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, 2, L.K, L.Kstride, d.precision);
     BinPlan b = make_bin_plan(d.grid, 0, d.nbins, 0, d.los, 1.0f);
-    FftPlan &p = get_plan(ctx, 1, L.Np, nmaps, L.Kstride);
+    FftPlan &p = maps_plan(ctx, sh, L.Np, d.grid, nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(nmaps);
     L.scale_exp = a.take<float>(nmaps);
@@ -865,15 +935,17 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
             for (int i = 0; i < 3; ++i) fp.los[i] = (float)d->los[i];
             fp.shell_table = sh.table_d;
             fp.max_d2 = sh.max_d2;
+            fp.shell_d2 = sh.d2range_d;
             TimedScope ts(&ctx->timer, "pbi4_filter", s);
-            launch_filter(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,
-                          sh.nmodes_d, L.spectra + f * nb * spec, L.scale_exp + f * nb, s);
+            if (maps_fused(L.Np, d->grid, sh))
+                launch_filter_z(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,
+                                get_twiddle(ctx, L.Np), L.spectra + f * nb * spec,
+                                L.scale_exp + f * nb, s);
+            else
+                launch_filter(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,
+                              sh.nmodes_d, L.spectra + f * nb * spec, L.scale_exp + f * nb, s);
         }
-        FftPlan &p = get_plan(ctx, 1, L.Np, 2 * nb, L.Kstride);
-        {
-            TimedScope ts(&ctx->timer, "pbi4_c2r", s);
-            fft_exec(p, L.spectra, L.maps, L.fftwork, s);
-        }
+        maps_c2r(ctx, sh, L.Np, d->grid, 2 * nb, L.Kstride, L.spectra, L.maps, L.fftwork, s);
         launch_gemm(L.g, L.maps, L.gemm_partials, s, &ctx->timer);
         launch_gemm_reduce(L.g, L.gemm_partials, L.C, s);
         // kf3 = kf**3 in REAL(4) (bispectrumI4.f90:569)

@@ -110,10 +110,14 @@ struct FilterPlan {
     float los[3];
     const int16_t *shell_table;
     int max_d2;
+    const int32_t *shell_d2;  // per shell: smallest and largest d2 (the fused z filter)
     const float2 *f2, *f4;    // explicit delta_2, delta_4 fields, or null for L_l(mu) delta
 };
 void launch_filter(const FilterPlan &f, const float2 *delta, const double *shell_sum,
                    const double *nmodes, float2 *spectra, float *scale_exp, hipStream_t s);
+// Band-limited maps grid (3R < N' < N): shell filter fused with the inverse transform over z.
+void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *shell_sum,
+                     const float2 *tw, float2 *spectra, float *scale_exp, hipStream_t s);
 
 // ---------------------------------------------------------------- triangles (triangle.hip)
 struct GemmPlan {

@@ -421,6 +421,99 @@ __global__ void k_filter(FilterPlan f, const float2 *delta, const float *scale_e
     }
 }
 
+// Shell filter fused with the inverse transform over z, for a maps grid N' below the delta grid:
+// every shell mode has |q_c| <= R < N'/3, and along one (x', q_y) line only the q_z of shell s
+// with d2 = x'^2 + q_y^2 + q_z^2 inside the shell's d2 range are nonzero: two runs of q_z,
+// -qhi..-qlo and qlo..qhi.  One thread owns one line, one shell, its NELL maps and ZC
+// consecutive planes z, and sums those modes directly,
+// A_l(z) = sum_{q_z in S_s} H_l(q) e^{+2 pi i q_z z / N'}, with H_l exactly as k_filter forms it.
+// The phase is stepped by one complex product per plane from tw[k] = e^{2 pi i k / N'}.
+// Output: the half spectra [m][z][y][x'] after the z transform (m = l * nshell + s - 1); a
+// batched 2D c2r over (y, x) completes the maps.  Lines outside the band get zeros.
+__device__ __forceinline__ int isqrt_floor(int v) {
+    int r = (int)sqrtf((float)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+template <int ZC, int NELL>
+__global__ __launch_bounds__(256) void k_filter_z(FilterPlan f, const float2 *delta, const float *scale_exp,
+                           const float2 *tw, float2 *out) {
+    const int Np = f.Np, hp = Np / 2 + 1;
+    const int line = blockIdx.x * blockDim.x + threadIdx.x;  // y * hp + x'
+    if (line >= Np * hp) return;
+    const int s = blockIdx.y + 1, z0 = blockIdx.z * ZC;
+    const int x = line % hp, y = line / hp;
+    const int qy = y <= Np / 2 ? y : y - Np;
+    const int R = f.R;
+    float2 acc[NELL][ZC];
+#pragma unroll
+    for (int l = 0; l < NELL; ++l)
+#pragma unroll
+        for (int j = 0; j < ZC; ++j) acc[l][j] = make_float2(0.0f, 0.0f);
+    int n1 = 0, n2 = 0, qhi = 0, qlo2 = 1;
+    const int r2 = x * x + qy * qy;
+    if (x <= R && qy <= R && qy >= -R) {
+        const int a = f.shell_d2[2 * (s - 1)], b = f.shell_d2[2 * (s - 1) + 1];
+        if (a <= b && b >= r2) {
+            qhi = min(isqrt_floor(b - r2), R);
+            int qlo = 0;
+            if (a > r2) {
+                qlo = isqrt_floor(a - r2);
+                if (qlo * qlo < a - r2) ++qlo;
+            }
+            if (qlo <= qhi) {
+                n1 = qhi - qlo + 1;
+                qlo2 = max(qlo, 1);
+                n2 = max(qhi - qlo2 + 1, 0);
+            }
+        }
+    }
+    const int n = n1 + n2;
+    if (n > 0) {
+        const int N = f.N, Y = (qy + N) % N;
+        const float sc = ldexpf(1.0f, (int)scale_exp[s - 1]);
+        // every d2 of the shell's range belongs to it (shells are |q| intervals; the host checks
+        // this before taking this path), so the runs hold exactly the shell's modes
+        for (int t = 0; t < n; ++t) {
+            const int qz = t < n1 ? t - qhi : qlo2 + (t - n1);
+            const int Z = (qz + N) % N;
+            float2 H[NELL];
+            H[0] = cscale(herm_at(delta, N, x, Y, Z), sc);
+            if (NELL == 3) {
+                const int d2 = r2 + qz * qz;
+                const float lk = f.los[0] * x + f.los[1] * qy + f.los[2] * qz;
+                const float mu2 = d2 > 0 ? lk * lk / (float)d2 : 0.0f;
+                H[1] = f.f2 ? cscale(herm_at(f.f2, N, x, Y, Z), sc)
+                            : cscale(H[0], -0.5f + 1.5f * mu2);
+                H[2] = f.f4 ? cscale(herm_at(f.f4, N, x, Y, Z), sc)
+                            : cscale(H[0], 0.375f + mu2 * (-3.75f + 4.375f * mu2));
+            }
+            const int step = (qz + Np) % Np;
+            float2 w = tw[(int)(((int64_t)step * z0) % Np)];
+            const float2 ws = tw[step];
+#pragma unroll
+            for (int j = 0; j < ZC; ++j) {
+#pragma unroll
+                for (int l = 0; l < NELL; ++l) {
+                    acc[l][j].x = fmaf(H[l].x, w.x, fmaf(-H[l].y, w.y, acc[l][j].x));
+                    acc[l][j].y = fmaf(H[l].x, w.y, fmaf(H[l].y, w.x, acc[l][j].y));
+                }
+                w = make_float2(w.x * ws.x - w.y * ws.y, w.x * ws.y + w.y * ws.x);
+            }
+        }
+    }
+    const int64_t plane = (int64_t)Np * hp;
+#pragma unroll
+    for (int l = 0; l < NELL; ++l) {
+        const int64_t m = (int64_t)l * f.nshell + s - 1;
+#pragma unroll
+        for (int j = 0; j < ZC; ++j)
+            if (z0 + j < Np) out[(m * Np + z0 + j) * plane + line] = acc[l][j];
+    }
+}
+
 // Power-of-two shell scales: e_s = -round(log2(sqrt(sum_{q in S_s} |H_q|^2))), so each map has
 // O(1) RMS in FP32 / BF16 whatever the catalogue normalisation.
 __global__ void k_shell_scales(int nshell, const double *shell_sum, float *scale_exp) {
@@ -646,6 +739,30 @@ void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, do
     if (n <= 0) return;
     hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)n), dim3(kFinThreads), 0, s, b, partials, kF3,
                        pk_out, shell_sum);
+    PBI4_HIP(hipGetLastError());
+}
+
+
+void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *shell_sum,
+                     const float2 *tw, float2 *spectra, float *scale_exp, hipStream_t s) {
+    PBI4_CHECK(3 * f.R < f.Np && f.Np < f.N, PBI4_EINVAL,
+               "the fused z filter needs a band-limited maps grid");
+    hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)ceil_div(f.nshell, 64)), dim3(64), 0, s,
+                       f.nshell, shell_sum, scale_exp);
+    PBI4_HIP(hipGetLastError());
+    PBI4_CHECK(f.shell_d2 != nullptr, PBI4_EINVAL, "the fused z filter needs the shell d2 ranges");
+    const int lines = f.Np * (f.Np / 2 + 1);
+    // planes per thread: 16 with three multipole maps (96 accumulator registers), else 32
+    if (f.nell == 3)
+        hipLaunchKernelGGL((k_filter_z<16, 3>),
+                           dim3((unsigned)ceil_div(lines, 256), (unsigned)f.nshell,
+                                (unsigned)ceil_div(f.Np, 16)),
+                           dim3(256), 0, s, f, delta, scale_exp, tw, spectra);
+    else
+        hipLaunchKernelGGL((k_filter_z<32, 1>),
+                           dim3((unsigned)ceil_div(lines, 256), (unsigned)f.nshell,
+                                (unsigned)ceil_div(f.Np, 32)),
+                           dim3(256), 0, s, f, delta, scale_exp, tw, spectra);
     PBI4_HIP(hipGetLastError());
 }
 

@@ -352,8 +352,11 @@ def _bk_close(mine, ref, rows, rtol=1e-4):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("multipoles", [False, True])
-@pytest.mark.parametrize("maps_grid", [0, 32])
+@pytest.mark.parametrize("maps_grid", [0, 30, 32])
 def test_bispectrum(ctx, prec, multipoles, maps_grid):
+    """maps_grid 0: the band-limited grid (z transform fused into the shell filter, one 2D c2r
+    over all maps); 30: also fused, but 30^3 is not a multiple of 64, so the maps' stride differs
+    from 30^3 and the 2D c2r runs per map; 32: the full grid (3D c2r)."""
     N, n, box, nb, kb, kc = 32, 60000, 1000.0, 6, 1.0, 1.0
     pos = _pos(n, 77)
     g = ctx.assign(_dev(pos), N, 4, True)
@@ -374,6 +377,26 @@ def test_bispectrum(ctx, prec, multipoles, maps_grid):
     else:
         ref = O.measure_bk(nb, N, kb, kc, True, kf, itot, n, counts, dcl)
     _bk_close(bout, ref, 11 if multipoles else 9)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_bispectrum_explicit_multipole_fields_fused_vs_full_grid(ctx, prec):
+    """Explicit delta_2 / delta_4 fields (here the densities of two other catalogues) through the
+    fused z filter on the band-limited grid agree with the shell filter + 3D c2r on the full grid."""
+    N, n, box, nb, kb, kc = 32, 60000, 1000.0, 6, 1.0, 1.0
+    fields = [ctx.density(ctx.assign(_dev(_pos(n, seed)), N, 4, True), N, 4, True, box, n)
+              for seed in (77, 78, 79)]
+    itot = O.itot_count(nb, kb, kc, True)
+    ntr = ctx.count_triangles(nb, N, kb, kc, True, False, itot)
+    counts = _dev((ntr.astype(np.float64) * N ** 3).astype(np.float32))
+    kf = np.float32(2 * np.pi / box)
+    out = {}
+    for mg in (0, N):
+        desc = ctx.bk_desc(nb, N, kb, kc, True, kf, itot, n, True, (0.0, 0.0, 1.0), prec, mg)
+        if mg == 0:
+            assert ctx.maps_grid(desc) < N
+        out[mg] = ctx.bk(fields[0], counts, desc, delta2=fields[1], delta4=fields[2]).cpu().numpy().T
+    _bk_close(out[0], out[N], 11)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
