@@ -426,8 +426,9 @@ __global__ void k_filter(FilterPlan f, const float2 *delta, const float *scale_e
 // with d2 = x'^2 + q_y^2 + q_z^2 inside the shell's d2 range are nonzero: two runs of q_z,
 // -qhi..-qlo and qlo..qhi.  One thread owns one line, one shell, its NELL maps and ZC
 // consecutive planes z, and sums those modes directly,
-// A_l(z) = sum_{q_z in S_s} H_l(q) e^{+2 pi i q_z z / N'}, with H_l exactly as k_filter forms it.
-// The phase is stepped by one complex product per plane from tw[k] = e^{2 pi i k / N'}.
+// A_l(z) = sum_{q_z in S_s} H_l(q) e^{+2 pi i q_z z / N'}, with H_l exactly as k_filter forms it,
+// the modes +q_z and -q_z as one pair sharing one phase.  The phase is stepped by one complex
+// product per plane from tw[k] = e^{2 pi i k / N'}.
 // Output: the half spectra [m][z][y][x'] after the z transform (m = l * nshell + s - 1); a
 // batched 2D c2r over (y, x) completes the maps.  Lines outside the band get zeros.
 __device__ __forceinline__ int isqrt_floor(int v) {
@@ -452,37 +453,27 @@ __global__ __launch_bounds__(256) void k_filter_z(FilterPlan f, const float2 *de
     for (int l = 0; l < NELL; ++l)
 #pragma unroll
         for (int j = 0; j < ZC; ++j) acc[l][j] = make_float2(0.0f, 0.0f);
-    int n1 = 0, n2 = 0, qhi = 0, qlo2 = 1;
+    int qlo = 0, qhi = -1;
     const int r2 = x * x + qy * qy;
     if (x <= R && qy <= R && qy >= -R) {
         const int a = f.shell_d2[2 * (s - 1)], b = f.shell_d2[2 * (s - 1) + 1];
         if (a <= b && b >= r2) {
             qhi = min(isqrt_floor(b - r2), R);
-            int qlo = 0;
             if (a > r2) {
                 qlo = isqrt_floor(a - r2);
                 if (qlo * qlo < a - r2) ++qlo;
             }
-            if (qlo <= qhi) {
-                n1 = qhi - qlo + 1;
-                qlo2 = max(qlo, 1);
-                n2 = max(qhi - qlo2 + 1, 0);
-            }
         }
     }
-    const int n = n1 + n2;
-    if (n > 0) {
+    if (qlo <= qhi) {
         const int N = f.N, Y = (qy + N) % N;
         const float sc = ldexpf(1.0f, (int)scale_exp[s - 1]);
         // every d2 of the shell's range belongs to it (shells are |q| intervals; the host checks
         // this before taking this path), so the runs hold exactly the shell's modes
-        for (int t = 0; t < n; ++t) {
-            const int qz = t < n1 ? t - qhi : qlo2 + (t - n1);
+        auto mode = [&](int qz, int d2, float2 *H) {
             const int Z = (qz + N) % N;
-            float2 H[NELL];
             H[0] = cscale(herm_at(delta, N, x, Y, Z), sc);
             if (NELL == 3) {
-                const int d2 = r2 + qz * qz;
                 const float lk = f.los[0] * x + f.los[1] * qy + f.los[2] * qz;
                 const float mu2 = d2 > 0 ? lk * lk / (float)d2 : 0.0f;
                 H[1] = f.f2 ? cscale(herm_at(f.f2, N, x, Y, Z), sc)
@@ -490,15 +481,32 @@ __global__ __launch_bounds__(256) void k_filter_z(FilterPlan f, const float2 *de
                 H[2] = f.f4 ? cscale(herm_at(f.f4, N, x, Y, Z), sc)
                             : cscale(H[0], 0.375f + mu2 * (-3.75f + 4.375f * mu2));
             }
-            const int step = (qz + Np) % Np;
-            float2 w = tw[(int)(((int64_t)step * z0) % Np)];
-            const float2 ws = tw[step];
+        };
+        // +q_z and -q_z together: with w = e^{i phi q_z z}, H+ w + H- conj(w) =
+        // (S.x w.x - D.y w.y, S.y w.x + D.x w.y), S = H+ + H-, D = H+ - H-
+        for (int qz = qlo; qz <= qhi; ++qz) {
+            const int d2 = r2 + qz * qz;
+            float2 Hp[NELL], Hm[NELL], S[NELL], D[NELL];
+            mode(qz, d2, Hp);
+            if (qz > 0) {
+                mode(-qz, d2, Hm);
+            } else {
+#pragma unroll
+                for (int l = 0; l < NELL; ++l) Hm[l] = make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int l = 0; l < NELL; ++l) {
+                S[l] = make_float2(Hp[l].x + Hm[l].x, Hp[l].y + Hm[l].y);
+                D[l] = make_float2(Hp[l].x - Hm[l].x, Hp[l].y - Hm[l].y);
+            }
+            float2 w = tw[(int)(((int64_t)qz * z0) % Np)];
+            const float2 ws = tw[qz];
 #pragma unroll
             for (int j = 0; j < ZC; ++j) {
 #pragma unroll
                 for (int l = 0; l < NELL; ++l) {
-                    acc[l][j].x = fmaf(H[l].x, w.x, fmaf(-H[l].y, w.y, acc[l][j].x));
-                    acc[l][j].y = fmaf(H[l].x, w.y, fmaf(H[l].y, w.x, acc[l][j].y));
+                    acc[l][j].x = fmaf(S[l].x, w.x, fmaf(-D[l].y, w.y, acc[l][j].x));
+                    acc[l][j].y = fmaf(S[l].y, w.x, fmaf(D[l].x, w.y, acc[l][j].y));
                 }
                 w = make_float2(w.x * ws.x - w.y * ws.y, w.x * ws.y + w.y * ws.x);
             }
@@ -752,16 +760,16 @@ void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *she
     PBI4_HIP(hipGetLastError());
     PBI4_CHECK(f.shell_d2 != nullptr, PBI4_EINVAL, "the fused z filter needs the shell d2 ranges");
     const int lines = f.Np * (f.Np / 2 + 1);
-    // planes per thread: 16 with three multipole maps (96 accumulator registers), else 32
+    // planes per thread: 20 with three multipole maps (120 accumulator registers), else 25
     if (f.nell == 3)
-        hipLaunchKernelGGL((k_filter_z<16, 3>),
+        hipLaunchKernelGGL((k_filter_z<20, 3>),
                            dim3((unsigned)ceil_div(lines, 256), (unsigned)f.nshell,
-                                (unsigned)ceil_div(f.Np, 16)),
+                                (unsigned)ceil_div(f.Np, 20)),
                            dim3(256), 0, s, f, delta, scale_exp, tw, spectra);
     else
-        hipLaunchKernelGGL((k_filter_z<32, 1>),
+        hipLaunchKernelGGL((k_filter_z<25, 1>),
                            dim3((unsigned)ceil_div(lines, 256), (unsigned)f.nshell,
-                                (unsigned)ceil_div(f.Np, 32)),
+                                (unsigned)ceil_div(f.Np, 25)),
                            dim3(256), 0, s, f, delta, scale_exp, tw, spectra);
     PBI4_HIP(hipGetLastError());
 }
