@@ -671,7 +671,7 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
         const double cf = (d->interlacing ? 1.0 / (4.0 * kF3) : 1.0 / kF3) / d->npart;
         TimedScope ts(&ctx->timer, "pbi4_combine", s);
         launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order), (float)cf, w.G,
-                       nullptr, static_cast<float2 *>(delta), s);
+                       static_cast<float2 *>(delta), s);
     });
 }
 

@@ -71,8 +71,9 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
                 float *grids, Arena &ws, hipStream_t s, KernelTimer *timer);
 
 // ---------------------------------------------------------------- spectra (spectrum.hip)
+// Interlaced: G1 is the full N^3 transform of grid1 + i grid2; single grid: the r2c half spectrum.
 void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
-                    const float2 *G2, float2 *delta, hipStream_t s);
+                    float2 *delta, hipStream_t s);
 
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
                        float2 *out, hipStream_t s);

@@ -14,32 +14,13 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 
 // ------------------------------------------------------------------ fcomb (powerI4.f90:251-378)
 
-// F(X,Y,Z) = G1 + i G2 of the reference's c2c transform of (grid1 + i grid2), read from the two
-// r2c half spectra (H = N/2+1 entries along x): G(k) = conj(G(-k)) for X > N/2.
-__device__ __forceinline__ float2 read_F(int N, int H, const float2 *G1, const float2 *G2, int X,
-                                         int Y, int Z) {
-    float2 g1, g2;
-    if (X <= N / 2) {
-        const int64_t o = ((int64_t)Z * N + Y) * H + X;
-        g1 = G1[o];
-        g2 = G2[o];
-    } else {
-        const int64_t o = ((int64_t)((N - Z) % N) * N + (N - Y) % N) * H + (N - X);
-        g1 = conjf2(G1[o]);
-        g2 = conjf2(G2[o]);
-    }
-    return make_float2(g1.x - g2.y, g1.y + g2.x);
-}
-
 // One thread per iteration (ix, iy, iz) in [0, N/2]^3 of the reference's interlaced loop.  The
 // iteration owns the <= 8 cells {ix,icx} x {iy,icy} x {iz,icz}; their eight ordered stores are
 // replayed in registers (a later store to the same cell wins, later stores read earlier ones),
 // then the cells in the x <= N/2 half are written.  Code bits: 1 = icx, 2 = icy, 4 = icz.
-// FULL: G1 is the full N^3 transform F of grid1 + i grid2 itself (one complex FFT); otherwise G1
-// and G2 are the half spectra of the two grids' real FFTs and F is rebuilt from them.
-template <bool FULL>
+// F is the full N^3 transform of grid1 + i grid2 (one complex FFT).
 __global__ void k_combine_interlaced(int N, float theta, const float *wtab, float cf,
-                                     const float2 *G1, const float2 *G2, float2 *delta) {
+                                     const float2 *F, float2 *delta) {
     const int h = N / 2 + 1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)h * h * h) return;
@@ -47,41 +28,12 @@ __global__ void k_combine_interlaced(int N, float theta, const float *wtab, floa
     const int icx = (N - ix) % N, icy = (N - iy) % N, icz = (N - iz) % N;
     const int mask = (ix == icx ? 1 : 0) | (iy == icy ? 2 : 0) | (iz == icz ? 4 : 0);
 
-    // The 8 values F(X, Y, Z), X in {ix, icx}, Y in {iy, icy}, Z in {iz, icz}: read directly from
-    // the full transform, or rebuilt from the 4 half-spectrum entries (ix, Y, Z) of each real
-    // transform: for X = icx > N/2, F(icx, Y, Z) = conj of the entry at the mirrored (Y, Z)
-    // (read_F).  Loaded unconditionally as one batch.
+    // The 8 values F(X, Y, Z), X in {ix, icx}, Y in {iy, icy}, Z in {iz, icz}, loaded as one batch.
     float2 orig[8];
-    if (FULL) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int X = (k & 1) ? icx : ix, Y = (k & 2) ? icy : iy, Z = (k & 4) ? icz : iz;
-            orig[k] = G1[((int64_t)Z * N + Y) * N + X];
-        }
-    } else {
-    float2 g1[2][2], g2[2][2];
-#pragma unroll
-    for (int yb = 0; yb < 2; ++yb)
-#pragma unroll
-        for (int zb = 0; zb < 2; ++zb) {
-            const int64_t o = ((int64_t)(zb ? icz : iz) * N + (yb ? icy : iy)) * h + ix;
-            g1[yb][zb] = G1[o];
-            g2[yb][zb] = G2[o];
-        }
-    const bool mirror = icx > N / 2;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const int xb = k & 1, yb = (k >> 1) & 1, zb = (k >> 2) & 1;
-        float2 a, b;
-        if (xb && mirror) {
-            a = conjf2(g1[1 - yb][1 - zb]);
-            b = conjf2(g2[1 - yb][1 - zb]);
-        } else {
-            a = g1[yb][zb];
-            b = g2[yb][zb];
-        }
-        orig[k] = make_float2(a.x - b.y, a.y + b.x);
-    }
+        const int X = (k & 1) ? icx : ix, Y = (k & 2) ? icy : iy, Z = (k & 4) ? icz : iz;
+        orig[k] = F[((int64_t)Z * N + Y) * N + X];
     }
 
     auto cm = [&](int s) {  // i * exp(-i theta s)
@@ -708,17 +660,13 @@ void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, 
 }
 
 void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
-                    const float2 *G2, float2 *delta, hipStream_t s) {
+                    float2 *delta, hipStream_t s) {
     const int h = N / 2 + 1;
     if (interlacing) {
         const float theta = (float)((double)kTwoPiF32 / 2.0 / (double)N);
         const int64_t n = (int64_t)h * h * h;
-        if (G2)
-            hipLaunchKernelGGL(k_combine_interlaced<false>, dim3((unsigned)ceil_div(n, 256)), dim3(256),
-                               0, s, N, theta, wtab, cf, G1, G2, delta);
-        else
-            hipLaunchKernelGGL(k_combine_interlaced<true>, dim3((unsigned)ceil_div(n, 256)), dim3(256),
-                               0, s, N, theta, wtab, cf, G1, G2, delta);
+        hipLaunchKernelGGL(k_combine_interlaced, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                           N, theta, wtab, cf, G1, delta);
     } else {
         const int64_t n = (int64_t)N * N * h;
         hipLaunchKernelGGL(k_combine_single, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, N,
