@@ -180,10 +180,11 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
     if (kind == 3) {
         // c2r over (y, x) of `batch` N x N planes: half spectra [y][x'] in, real [y][x] out
         const size_t h = N / 2 + 1;
-        size_t in_strides[2] = {1, h}, out_strides[2] = {1, (size_t)N};
+        const size_t ld = ext ? (size_t)ext : h;
+        size_t in_strides[2] = {1, ld}, out_strides[2] = {1, (size_t)N};
         fft_check(rocfft_plan_description_set_data_layout(
                       desc, rocfft_array_type_hermitian_interleaved, rocfft_array_type_real, nullptr,
-                      nullptr, 2, in_strides, h * N, 2, out_strides, (size_t)N * N),
+                      nullptr, 2, in_strides, ld * N, 2, out_strides, (size_t)N * N),
                   "rocfft_plan_description_set_data_layout");
     }
     if (kind == 1) {
@@ -262,11 +263,18 @@ float2 *get_twiddle(pbi4_ctx *ctx, int Np) {
 // filter writes half spectra and a 3D c2r transforms them.
 bool maps_fused(int Np, int grid, const ShellInfo &sh) { return Np < grid && sh.contiguous; }
 
+// Row stride of the maps' half spectra: N'/2 + 1, padded to a multiple of 64 entries on the
+// fused path (512-byte rows for the 2D c2r's column pass).
+int maps_ld(int Np, int grid, const ShellInfo &sh) {
+    const int h = Np / 2 + 1;
+    return maps_fused(Np, grid, sh) ? (h + 63) / 64 * 64 : h;
+}
+
 FftPlan &maps_plan(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
                    int64_t Kstride) {
     if (!maps_fused(Np, grid, sh)) return get_plan(ctx, 1, Np, nmaps, Kstride);
     const bool one = Kstride == (int64_t)Np * Np * Np;
-    return get_plan(ctx, 3, Np, one ? nmaps * Np : Np, 0);
+    return get_plan(ctx, 3, Np, one ? nmaps * Np : Np, maps_ld(Np, grid, sh));
 }
 
 void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, int64_t Kstride,
@@ -277,7 +285,7 @@ void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, i
         fft_exec(p, spectra, maps, work, s);
         return;
     }
-    const size_t spec = (size_t)Np * Np * (Np / 2 + 1);
+    const size_t spec = (size_t)Np * Np * maps_ld(Np, grid, sh);
     for (int m = 0; m < nmaps; ++m) fft_exec(p, spectra + m * spec, maps + m * Kstride, work, s);
 }
 
@@ -484,7 +492,7 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a, int nbins_pk 
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(d.nbins);
     L.scale_exp = a.take<float>(d.nbins);
-    L.spectra = a.take<float2>((size_t)L.nmaps * L.Np * L.Np * (L.Np / 2 + 1));
+    L.spectra = a.take<float2>((size_t)L.nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh));
     L.maps = a.take<float>((size_t)L.nmaps * L.Kstride);
     L.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
     L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
@@ -506,6 +514,7 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
     f.shell_table = sh.table_d;
     f.max_d2 = sh.max_d2;
     f.shell_d2 = sh.d2range_d;
+    f.ld = maps_ld(L.Np, d->grid, sh);
     f.f2 = static_cast<const float2 *>(delta2);
     f.f4 = static_cast<const float2 *>(delta4);
     PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
@@ -568,7 +577,7 @@ BkCrossLayout bk_cross_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(nmaps);
     L.scale_exp = a.take<float>(nmaps);
-    L.spectra = a.take<float2>((size_t)nmaps * L.Np * L.Np * (L.Np / 2 + 1));
+    L.spectra = a.take<float2>((size_t)nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh));
     L.maps = a.take<float>((size_t)nmaps * L.Kstride);
     L.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
     L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
@@ -917,7 +926,7 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
         Arena a(ws, ws_bytes);
         BkCrossLayout L = bk_cross_layout(ctx, *d, a);
         const int nb = d->nbins;
-        const size_t spec = (size_t)L.Np * L.Np * (L.Np / 2 + 1);
+        const size_t spec = (size_t)L.Np * L.Np * maps_ld(L.Np, d->grid, sh);
         const void *fields[2] = {delta1, delta2};
         for (int f = 0; f < 2; ++f) {
             // shell powers (Parseval) -> map scales; shell filter into maps f*nb .. f*nb+nb-1
@@ -936,6 +945,7 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
             fp.shell_table = sh.table_d;
             fp.max_d2 = sh.max_d2;
             fp.shell_d2 = sh.d2range_d;
+            fp.ld = maps_ld(L.Np, d->grid, sh);
             TimedScope ts(&ctx->timer, "pbi4_filter", s);
             if (maps_fused(L.Np, d->grid, sh))
                 launch_filter_z(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,

@@ -112,6 +112,7 @@ struct FilterPlan {
     const int16_t *shell_table;
     int max_d2;
     const int32_t *shell_d2;  // per shell: smallest and largest d2 (the fused z filter)
+    int ld;                   // the fused z filter's output row stride (>= N'/2 + 1)
     const float2 *f2, *f4;    // explicit delta_2, delta_4 fields, or null for L_l(mu) delta
 };
 void launch_filter(const FilterPlan &f, const float2 *delta, const double *shell_sum,

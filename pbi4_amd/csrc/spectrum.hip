@@ -381,8 +381,9 @@ __global__ void k_filter(FilterPlan f, const float2 *delta, const float *scale_e
 // A_l(z) = sum_{q_z in S_s} H_l(q) e^{+2 pi i q_z z / N'}, with H_l exactly as k_filter forms it,
 // the modes +q_z and -q_z as one pair sharing one phase.  The phase is stepped by one complex
 // product per plane from tw[k] = e^{2 pi i k / N'}.
-// Output: the half spectra [m][z][y][x'] after the z transform (m = l * nshell + s - 1); a
-// batched 2D c2r over (y, x) completes the maps.  Lines outside the band get zeros.
+// Output: the half spectra [m][z][y][x'] after the z transform (m = l * nshell + s - 1), rows of
+// f.ld >= N'/2 + 1 entries; a batched 2D c2r over (y, x) completes the maps.  Lines outside the
+// band get zeros.
 __device__ __forceinline__ int isqrt_floor(int v) {
     int r = (int)sqrtf((float)v);
     while (r * r > v) --r;
@@ -464,13 +465,14 @@ __global__ __launch_bounds__(256) void k_filter_z(FilterPlan f, const float2 *de
             }
         }
     }
-    const int64_t plane = (int64_t)Np * hp;
+    const int64_t plane = (int64_t)Np * f.ld;   // rows padded to f.ld entries (never read)
+    const int64_t cell = (int64_t)y * f.ld + x;
 #pragma unroll
     for (int l = 0; l < NELL; ++l) {
         const int64_t m = (int64_t)l * f.nshell + s - 1;
 #pragma unroll
         for (int j = 0; j < ZC; ++j)
-            if (z0 + j < Np) out[(m * Np + z0 + j) * plane + line] = acc[l][j];
+            if (z0 + j < Np) out[(m * Np + z0 + j) * plane + cell] = acc[l][j];
     }
 }
 
@@ -707,6 +709,7 @@ void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *she
                        f.nshell, shell_sum, scale_exp);
     PBI4_HIP(hipGetLastError());
     PBI4_CHECK(f.shell_d2 != nullptr, PBI4_EINVAL, "the fused z filter needs the shell d2 ranges");
+    PBI4_CHECK(f.ld > f.Np / 2, PBI4_EINVAL, "the fused z filter's row stride is too small");
     const int lines = f.Np * (f.Np / 2 + 1);
     // planes per thread: 20 with three multipole maps (120 accumulator registers), else 25
     if (f.nell == 3)
