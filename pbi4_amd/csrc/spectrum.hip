@@ -138,14 +138,20 @@ struct ModeLoads {
     int bin, shell;
 };
 
+// EXTRA: some of the optional fields (explicit delta_2, delta_4, cross field) are present.
+template <bool EXTRA>
 __device__ __forceinline__ ModeLoads mode_loads(const BinPlan &b, const float2 *delta, int64_t row,
                                                 int h, int x, int aky, int akz) {
     ModeLoads m;
     const int64_t o = row * h + x;
     m.d = delta[o];
-    m.e2 = (b.f2 ? b.f2 : delta)[o];
-    m.e4 = (b.f4 ? b.f4 : delta)[o];
-    m.ec = (b.cross ? b.cross : delta)[o];
+    if (EXTRA) {
+        m.e2 = (b.f2 ? b.f2 : delta)[o];
+        m.e4 = (b.f4 ? b.f4 : delta)[o];
+        m.ec = (b.cross ? b.cross : delta)[o];
+    } else {
+        m.e2 = m.e4 = m.ec = m.d;
+    }
     m.bin = b.nbins_pk > 0 ? (int)b.pk_table[((int64_t)akz * h + aky) * h + x] : 0;
     const int d2 = x * x + aky * aky + akz * akz;
     m.shell = b.nshell > 0 ? (int)b.shell_table[min(d2, b.max_d2)] : 0;
@@ -179,11 +185,13 @@ __device__ __forceinline__ ModeTerms mode_terms(const BinPlan &b, const ModeLoad
         t.key = bin;
         if (bin > 0) {
             auto dot = [&](float2 e) { return d.x * e.x + d.y * e.y; };   // Re(conj(d) e)
+            // hardware reciprocal and square root (1 ulp): the IEEE forms cost ~10 instructions
+            // each in this VALU-bound loop
             const float k2 = (float)d2;
             const float lk = b.los[0] * (float)x + lt;
-            const float mu2 = k2 > 0.0f ? lk * lk / k2 : 0.0f;
+            const float mu2 = k2 > 0.0f ? lk * lk * __builtin_amdgcn_rcpf(k2) : 0.0f;
             t.v[0] = 1.0f;
-            t.v[1] = b.kF * sqrtf(k2);
+            t.v[1] = b.kF * __builtin_amdgcn_sqrtf(k2);
             t.v[2] = (!b.multipoles && b.cross) ? dot(m.ec) : p;
             if (b.multipoles) {
                 t.v[3] = b.f2 ? dot(m.e2) : (-0.5f + 1.5f * mu2) * p;
@@ -208,6 +216,9 @@ __device__ __forceinline__ ModeTerms mode_terms(const BinPlan &b, const ModeLoad
 // grows with x, so P(k) bins and shell ids are non-decreasing in lane order and each run is
 // reduced in registers; run tails add into wave-private LDS accumulators (no atomics), the waves
 // are summed in a fixed order and each work-group writes one partial: deterministic.
+// NV: P(k) quantities scanned per mode, 5 with multipoles (count, k, P0, P2, P4), else 3.
+// EXTRA: explicit delta_2 / delta_4 / cross fields are read.
+template <int NV, bool EXTRA>
 __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *delta,
                                                      float *partials) {
     extern __shared__ float acc[];
@@ -226,23 +237,26 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
         const float lt = b.los[1] * (float)ky + b.los[2] * (float)kz;
         for (int x0 = 0; x0 < half; x0 += 64) {
             const int x = x0 + lane;
-            const ModeLoads ml = mode_loads(b, delta, row, h, min(x, half - 1), aky, akz);
-            const ModeLoads mn = mode_loads(b, delta, row, h, half, aky, akz);   // Nyquist mode
+            const ModeLoads ml = mode_loads<EXTRA>(b, delta, row, h, min(x, half - 1), aky, akz);
+            const ModeLoads mn = mode_loads<EXTRA>(b, delta, row, h, half, aky, akz);   // Nyquist
             ModeTerms t = mode_terms(b, ml, N, x, y, ky, aky, z, kz, akz, lt, x < half);
             const bool nyq_lane = x == half - 1;
             ModeTerms tn;
             if (nyq_lane) tn = mode_terms(b, mn, N, half, y, ky, aky, z, kz, akz, lt, true);
             const int last = min(63, half - 1 - x0);      // highest valid lane of this step
             if (b.nbins_pk > 0) {
-                seg_prefix_sum<5>(t.key, t.v, lane);
+                float v[NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q) v[q] = t.v[q];
+                seg_prefix_sum<NV>(t.key, v, lane);
                 const int next = __shfl_down(t.key, 1);
                 if (t.key > 0 && (lane == last || next != t.key)) {
 #pragma unroll
-                    for (int q = 0; q < 5; ++q) mine[q * b.nbins_pk + t.key - 1] += t.v[q];
+                    for (int q = 0; q < NV; ++q) mine[q * b.nbins_pk + t.key - 1] += v[q];
                 }
                 if (nyq_lane && tn.key > 0) {
 #pragma unroll
-                    for (int q = 0; q < 5; ++q) mine[q * b.nbins_pk + tn.key - 1] += tn.v[q];
+                    for (int q = 0; q < NV; ++q) mine[q * b.nbins_pk + tn.key - 1] += tn.v[q];
                 }
             }
             if (b.nshell > 0) {
@@ -684,10 +698,17 @@ size_t bin_partials_floats(const BinPlan &b) {
 void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStream_t s) {
     const size_t lds = sizeof(float) * kBinWaves * (5 * b.nbins_pk + b.nshell);
     PBI4_CHECK(lds <= 160 * 1024, PBI4_EUNSUPPORTED, "too many bins for the binning kernel");
-    if (lds > 64 * 1024)
-        PBI4_HIP(hipFuncSetAttribute((const void *)k_bin, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds));
-    hipLaunchKernelGGL(k_bin, dim3(b.nwg), dim3(kBinThreads), lds, s, b, delta, partials);
+    const bool extra = b.f2 || b.f4 || b.cross;
+    auto launch = [&](auto kern) {
+        if (lds > 64 * 1024)
+            PBI4_HIP(hipFuncSetAttribute((const void *)kern,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3(b.nwg), dim3(kBinThreads), lds, s, b, delta, partials);
+    };
+    if (b.multipoles)
+        extra ? launch(k_bin<5, true>) : launch(k_bin<5, false>);
+    else
+        extra ? launch(k_bin<3, true>) : launch(k_bin<3, false>);
     PBI4_HIP(hipGetLastError());
 }
 
