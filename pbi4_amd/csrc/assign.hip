@@ -600,6 +600,26 @@ struct Slab {
     int g, zlo, zhi;
 };
 
+// The particle of step s for this lane, q of lane j X + s (j = lane / X): a lane of the same
+// quad, so a DPP quad permute (a VALU move) instead of an LDS permute on the pipe the atomics use.
+// X = 4: broadcast of quad lane s; X = 2: lanes (0, 1) from lane s, (2, 3) from lane 2 + s.
+template <int X>
+__device__ __forceinline__ uint32_t step_particle(uint32_t v, int s) {
+    const int iv = (int)v;
+    if constexpr (X == 1) {
+        return v;
+    } else if constexpr (X == 2) {
+        return (uint32_t)(s == 0 ? dpp_mov<0xA0>(iv) : dpp_mov<0xF5>(iv));
+    } else {
+        switch (s) {   // s is the same on the whole wave
+        case 0: return (uint32_t)dpp_mov<0x00>(iv);
+        case 1: return (uint32_t)dpp_mov<0x55>(iv);
+        case 2: return (uint32_t)dpp_mov<0xAA>(iv);
+        default: return (uint32_t)dpp_mov<0xFF>(iv);
+        }
+    }
+}
+
 template <int ORDER, int NG, int RC, bool SLAB, typename Acc>
 __device__ __forceinline__ void deposit_chunk(Acc *acc, int R_, int GS, int per, float fix,
                                               Slab sl, uint32_t c0, uint32_t cend,
@@ -633,7 +653,7 @@ __device__ __forceinline__ void deposit_chunk(Acc *acc, int R_, int GS, int per,
             const int src = j * X + s;
             uint32_t qq[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) qq[d] = (uint32_t)__shfl((int)q[m][d], src);
+            for (int d = 0; d < 3; ++d) qq[d] = step_particle<X>(q[m][d], s);
             if constexpr (X == 4) {
                 // src is the same on the quad: the quad stays whole for the broadcasts
                 if (cb + (uint32_t)(64 * m + src) >= cend) continue;
