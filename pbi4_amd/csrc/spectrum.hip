@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
 #pragma unroll
                 for (int q = 0; q < NV; ++q) v[q] = t.v[q];
                 seg_prefix_sum<NV>(t.key, v, lane);
-                const int next = __shfl_down(t.key, 1);
+                const int next = __builtin_amdgcn_mov_dpp(t.key, 0x130, 0xF, 0xF, false);  // lane + 1
                 if (t.key > 0 && (lane == last || next != t.key)) {
 #pragma unroll
                     for (int q = 0; q < NV; ++q) mine[q * b.nbins_pk + t.key - 1] += v[q];
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
             }
             if (b.nshell > 0) {
                 seg_prefix_sum<1>(t.skey, t.sv, lane);
-                const int next = __shfl_down(t.skey, 1);
+                const int next = __builtin_amdgcn_mov_dpp(t.skey, 0x130, 0xF, 0xF, false);
                 if (t.skey > 0 && (lane == last || next != t.skey))
                     mine[5 * b.nbins_pk + t.skey - 1] += t.sv[0];
                 if (nyq_lane && tn.skey > 0) mine[5 * b.nbins_pk + tn.skey - 1] += tn.sv[0];
