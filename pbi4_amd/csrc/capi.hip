@@ -108,6 +108,7 @@ struct pbi4_ctx {
     std::map<std::tuple<int, float, float, float, int>, PkTable> pk_tables;
     std::map<std::tuple<int, int, float, float>, std::unique_ptr<ShellInfo>> shells;
     std::map<std::tuple<int, int>, float *> windows;   // (N, order) -> device sinc^p table
+    std::map<int, float2 *> phases;                    // N -> device e^{-i theta s}, s in [-N, 3N/2]
     std::map<int, float2 *> twiddles;                  // N' -> device e^{2 pi i k / N'}, k < N'
     std::map<std::tuple<int, int, int>, FftPlan> plans; // (kind, N, batch/ext)
     std::map<std::tuple<int, int, float, float>, CountsCache> counts;
@@ -242,6 +243,21 @@ float *get_window(pbi4_ctx *ctx, int N, int order) {
         w[i] = (float)y;
     }
     return ctx->windows[key] = dev_upload(ctx, w);
+}
+
+// The interlacing phase of fcomb (powerI4.f90:251-378), e^{-i theta s} for s = ix +- iy +- iz in
+// [-N, 3N/2] at index s + N, theta = twopi / N / 2 from the reference's REAL(4) twopi, evaluated
+// in FP64 and rounded once.
+float2 *get_phase(pbi4_ctx *ctx, int N) {
+    auto it = ctx->phases.find(N);
+    if (it != ctx->phases.end()) return it->second;
+    const double th = (double)kTwoPiF32 / (double)N / 2.0;
+    std::vector<float2> t(5 * (size_t)N / 2 + 1);
+    for (int k = 0; k < (int)t.size(); ++k) {
+        const double a = th * (double)(k - N);
+        t[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+    }
+    return ctx->phases[N] = dev_upload(ctx, t);
 }
 
 // e^{2 pi i k / N'} for k < N', from FP64, with entry N'-k set to the exact conjugate of k.
@@ -679,7 +695,8 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
         const double kF3 = kf3_fcomb(d->box);
         const double cf = (d->interlacing ? 1.0 / (4.0 * kF3) : 1.0 / kF3) / d->npart;
         TimedScope ts(&ctx->timer, "pbi4_combine", s);
-        launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order), (float)cf, w.G,
+        launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order),
+                       d->interlacing ? get_phase(ctx, d->grid) : nullptr, (float)cf, w.G,
                        static_cast<float2 *>(delta), s);
     });
 }

@@ -71,9 +71,10 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
                 float *grids, Arena &ws, hipStream_t s, KernelTimer *timer);
 
 // ---------------------------------------------------------------- spectra (spectrum.hip)
-// Interlaced: G1 is the full N^3 transform of grid1 + i grid2; single grid: the r2c half spectrum.
-void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
-                    float2 *delta, hipStream_t s);
+// Interlaced: G1 is the full N^3 transform of grid1 + i grid2 and ph[s + N] = e^{-i theta s},
+// theta = pi / N, s in [-N, 3N/2]; single grid: G1 is the r2c half spectrum and ph is unused.
+void launch_combine(int N, int interlacing, const float *wtab, const float2 *ph, float cf,
+                    const float2 *G1, float2 *delta, hipStream_t s);
 
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
                        float2 *out, hipStream_t s);

@@ -19,12 +19,28 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // replayed in registers (a later store to the same cell wins, later stores read earlier ones),
 // then the cells in the x <= N/2 half are written.  Code bits: 1 = icx, 2 = icy, 4 = icz.
 // F is the full N^3 transform of grid1 + i grid2 (one complex FFT).
-__global__ void k_combine_interlaced(int N, float theta, const float *wtab, float cf,
+// Threads: first the iterations with x in [1, N/2) (x runs fastest), then those with x in
+// {0, N/2}, so the ordered-store replay, needed only where ix = icx, runs in waves of its own.
+// ph[s + N] = e^{-i theta s} for s in [-N, 3N/2]: cm(s) = i e^{-i theta s}, s = ix +- iy +- iz,
+// one table entry (rounded once from FP64) per phase.
+__global__ void k_combine_interlaced(int N, const float2 *ph, const float *wtab, float cf,
                                      const float2 *F, float2 *delta) {
-    const int h = N / 2 + 1;
+    const int h = N / 2 + 1, hin = N / 2 - 1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)h * h * h) return;
-    const int ix = (int)(t % h), iy = (int)((t / h) % h), iz = (int)(t / ((int64_t)h * h));
+    const int64_t nin = (int64_t)hin * h * h;
+    int ix, iy, iz;
+    if (t < nin) {
+        ix = 1 + (int)(t % hin);
+        const int64_t r = t / hin;
+        iy = (int)(r % h);
+        iz = (int)(r / h);
+    } else {
+        const int64_t u = t - nin;
+        if (u >= 2 * (int64_t)h * h) return;
+        ix = (u & 1) ? N / 2 : 0;
+        iy = (int)((u >> 1) % h);
+        iz = (int)((u >> 1) / h);
+    }
     const int icx = (N - ix) % N, icy = (N - iy) % N, icz = (N - iz) % N;
     const int mask = (ix == icx ? 1 : 0) | (iy == icy ? 2 : 0) | (iz == icz ? 4 : 0);
 
@@ -36,10 +52,9 @@ __global__ void k_combine_interlaced(int N, float theta, const float *wtab, floa
         orig[k] = F[((int64_t)Z * N + Y) * N + X];
     }
 
-    auto cm = [&](int s) {  // i * exp(-i theta s)
-        float sn, cs;
-        sincosf(-theta * (float)s, &sn, &cs);
-        return make_float2(-sn, cs);
+    auto cm = [&](int sidx) {  // i * e^{-i theta s}
+        const float2 v = ph[sidx + N];
+        return make_float2(-v.y, v.x);
     };
     auto comb = [&](float2 a, float2 b, float2 c) {  // a (1 - c) + conj(b) (1 + c)
         return cadd(cmul(a, make_float2(1.0f - c.x, -c.y)), cmul(conjf2(b), make_float2(1.0f + c.x, c.y)));
@@ -49,6 +64,19 @@ __global__ void k_combine_interlaced(int N, float theta, const float *wtab, floa
     const float2 c001 = cscale(comb(orig[4], orig[3], cm(ix + iy - iz)), cfac);
     const float2 c010 = cscale(comb(orig[2], orig[5], cm(ix - iy + iz)), cfac);
     const float2 c011 = cscale(comb(orig[6], orig[1], cm(ix - iy - iz)), cfac);
+    if (!(mask & 1)) {
+        // ix != icx: the odd codes' stores land in other cells, and of the even codes, stored in
+        // the order 0, 4, 2, 6, the last one that reaches cell k is code k | mask (mask: y, z)
+        auto val = [&](int code) {
+            return code == 0 ? c000 : code == 2 ? c010 : code == 4 ? c001 : c011;
+        };
+#pragma unroll
+        for (int k = 0; k < 8; k += 2) {
+            const int Y = (k & 2) ? icy : iy, Z = (k & 4) ? icz : iz;
+            delta[((int64_t)Z * N + Y) * h + ix] = val(k | mask);
+        }
+        return;
+    }
 
     float2 mem[8];
 #pragma unroll
@@ -675,14 +703,14 @@ void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, 
     PBI4_HIP(hipGetLastError());
 }
 
-void launch_combine(int N, int interlacing, const float *wtab, float cf, const float2 *G1,
-                    float2 *delta, hipStream_t s) {
+void launch_combine(int N, int interlacing, const float *wtab, const float2 *ph, float cf,
+                    const float2 *G1, float2 *delta, hipStream_t s) {
     const int h = N / 2 + 1;
     if (interlacing) {
-        const float theta = (float)((double)kTwoPiF32 / 2.0 / (double)N);
+        PBI4_CHECK(ph != nullptr, PBI4_EINVAL, "the interlaced combine needs the phase table");
         const int64_t n = (int64_t)h * h * h;
         hipLaunchKernelGGL(k_combine_interlaced, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
-                           N, theta, wtab, cf, G1, delta);
+                           N, ph, wtab, cf, G1, delta);
     } else {
         const int64_t n = (int64_t)N * N * h;
         hipLaunchKernelGGL(k_combine_single, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, N,
