@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_filter_z(FilterPlan f, const float2 *de
             H[0] = cscale(herm_at(delta, N, x, Y, Z), sc);
             if (NELL == 3) {
                 const float lk = f.los[0] * x + f.los[1] * qy + f.los[2] * qz;
-                const float mu2 = d2 > 0 ? lk * lk / (float)d2 : 0.0f;
+                const float mu2 = d2 > 0 ? lk * lk * __builtin_amdgcn_rcpf((float)d2) : 0.0f;
                 H[1] = f.f2 ? cscale(herm_at(f.f2, N, x, Y, Z), sc)
                             : cscale(H[0], -0.5f + 1.5f * mu2);
                 H[2] = f.f4 ? cscale(herm_at(f.f4, N, x, Y, Z), sc)
