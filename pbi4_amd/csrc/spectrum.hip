@@ -158,9 +158,9 @@ __device__ __forceinline__ void seg_prefix_sum(int key, float (&v)[NV], int lane
     seg_step<0x143>(key, v, lane >= 32);
 }
 
-// Raw inputs of one mode, loaded unconditionally (indices clamped into range, absent optional
-// fields read delta instead) so a step's loads issue as one batch; a guarded load becomes a
-// branch with a wait after it.
+// Raw inputs of one mode, loaded unconditionally (indices clamped into range; with some optional
+// field present the absent ones read delta, with none present they are not loaded) so a step's
+// loads issue as one batch; a guarded load becomes a branch with a wait after it.
 struct ModeLoads {
     float2 d, e2, e4, ec;
     int bin, shell;
