@@ -20,24 +20,44 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <omp.h>
 
 /* Fortran NINT for REAL(8): round half away from zero (gfortran lowers it to lround). */
 static inline long fnint(double x) { return lround(x); }
 /* Fortran MOD for the non-negative arguments used below. */
 static inline long fmodi(long a, long n) { return a % n; }
 
-/*
- * assign_single_grid (powerI4.f90:68-140) when interlacing == 0, out = real N^3
- * assign_double_grid (powerI4.f90:142-249) when interlacing != 0, out = complex N^3
- * Zero-fills `out` first, as the f2py intent(out) wrapper does.
- */
-void orc_assign(int grid, int nintp, int interlacing, int64_t npart, const double *pos,
-                const double *weight, double *out)
+/* First 0-based cell index (before the periodic wrap) and width of the stencil of one axis
+ * for the coordinate v = N pos + 1 (or v - 1/2 for the interlaced grid), as each order's
+ * branch below computes its indices; used only to skip particles that cannot touch a slab. */
+static inline void stencil_span(int nintp, double v, long *first, int *width)
+{
+    switch (nintp) {
+        case 4: *first = (long)v - 2; *width = 4; break;
+        case 3: *first = fnint(v) - 2; *width = 3; break;
+        case 2: *first = (long)v; *width = 2; break;
+        default: *first = fnint(v) - 1; *width = 1; break;
+    }
+}
+
+static inline int span_touches(long first, int width, long N, long zlo, long zhi)
+{
+    for (int k = 0; k < width; ++k) {
+        const long z = (first + k + N) % N;
+        if (z >= zlo && z < zhi) return 1;
+    }
+    return 0;
+}
+
+/* The particle loop of assign_single_grid / assign_double_grid restricted to the output
+ * z-planes [zlo, zhi): every particle is visited in catalogue order and only its contributions
+ * to those planes are added, so each cell receives exactly the additions of the serial loop,
+ * in the same order. */
+static void assign_slab(int grid, int nintp, int interlacing, int64_t npart, const double *pos,
+                        const double *weight, double *out, long zlo, long zhi)
 {
     const long N = grid;
-    const long V = N * N * N;
     const int nintph = (nintp >= 3) ? 3 : 1;
-    memset(out, 0, sizeof(double) * (size_t)V * (interlacing ? 2 : 1));
     double w1[3][4] = {{0}}, w2[3][4] = {{0}};
     long iv[3][4], jv[3][4];
     for (int64_t i = 0; i < npart; ++i) {
@@ -46,6 +66,17 @@ void orc_assign(int grid, int nintp, int interlacing, int64_t npart, const doubl
         for (int d = 0; d < 3; ++d) {
             vr[d] = (double)N * pos[(int64_t)d * npart + i] + 1.0;
             vt[d] = vr[d] - 0.5;
+        }
+        {
+            long f;
+            int w;
+            stencil_span(nintp, vr[2], &f, &w);
+            int hit = span_touches(f, w, N, zlo, zhi);
+            if (!hit && interlacing) {
+                stencil_span(nintp, vt[2], &f, &w);
+                hit = span_touches(f, w, N, zlo, zhi);
+            }
+            if (!hit) continue;
         }
         if (!interlacing) {
             if (nintp == 4) { /* PCS  powerI4.f90:87-100 */
@@ -93,6 +124,7 @@ void orc_assign(int grid, int nintp, int interlacing, int64_t npart, const doubl
             for (int ix = 0; ix < nintp; ++ix)
                 for (int iy = 0; iy < nintp; ++iy)
                     for (int iz = 0; iz < nintp; ++iz) {
+                        if (iv[2][iz] < zlo || iv[2][iz] >= zhi) continue;
                         long c = iv[0][ix] + N * (iv[1][iy] + N * iv[2][iz]);
                         out[c] = out[c] + w1[0][ix] * w1[1][iy] * w1[2][iz] * wt;
                     }
@@ -164,11 +196,38 @@ void orc_assign(int grid, int nintp, int interlacing, int64_t npart, const doubl
         for (int ix = 0; ix < nintp; ++ix)
             for (int iy = 0; iy < nintp; ++iy)
                 for (int iz = 0; iz < nintp; ++iz) {
-                    long c = iv[0][ix] + N * (iv[1][iy] + N * iv[2][iz]);
-                    out[2 * c] = out[2 * c] + w1[0][ix] * w1[1][iy] * w1[2][iz] * wt;
-                    long e = jv[0][ix] + N * (jv[1][iy] + N * jv[2][iz]);
-                    out[2 * e + 1] = out[2 * e + 1] + w2[0][ix] * w2[1][iy] * w2[2][iz] * wt;
+                    if (iv[2][iz] >= zlo && iv[2][iz] < zhi) {
+                        long c = iv[0][ix] + N * (iv[1][iy] + N * iv[2][iz]);
+                        out[2 * c] = out[2 * c] + w1[0][ix] * w1[1][iy] * w1[2][iz] * wt;
+                    }
+                    if (jv[2][iz] >= zlo && jv[2][iz] < zhi) {
+                        long e = jv[0][ix] + N * (jv[1][iy] + N * jv[2][iz]);
+                        out[2 * e + 1] = out[2 * e + 1] + w2[0][ix] * w2[1][iy] * w2[2][iz] * wt;
+                    }
                 }
+    }
+}
+
+/*
+ * assign_single_grid (powerI4.f90:68-140) when interlacing == 0, out = real N^3
+ * assign_double_grid (powerI4.f90:142-249) when interlacing != 0, out = complex N^3
+ * Zero-fills `out` first, as the f2py intent(out) wrapper does.  The output z-planes are split
+ * over nthreads threads (0 = OpenMP default); each thread runs the reference's particle loop
+ * in order for its own planes, so the result is bit-identical to the serial loop for any
+ * thread count.
+ */
+void orc_assign(int grid, int nintp, int interlacing, int64_t npart, const double *pos,
+                const double *weight, double *out, int nthreads)
+{
+    const long N = grid;
+    const long V = N * N * N;
+    memset(out, 0, sizeof(double) * (size_t)V * (interlacing ? 2 : 1));
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+    {
+        const long T = omp_get_num_threads(), t = omp_get_thread_num();
+        const long zlo = N * t / T, zhi = N * (t + 1) / T;
+        if (zlo < zhi) assign_slab(grid, nintp, interlacing, npart, pos, weight, out, zlo, zhi);
     }
 }
 

@@ -19,6 +19,7 @@ reference's operation order, Fortran NINT/INT/MOD semantics, twopi = REAL(4) 6.2
   shell index / k maps ..... bispectrumI4.f90:65-157
   measure_bk ............... bispectrumI4.f90:1-198
   measure_bk_multipoles .... bispectrumI4.f90:200-516
+  measure_bk_subset ........ the same, for a subset of triangles (maps of their shells only)
   count_triangles .......... bispectrumI4.f90:763-915 (FFT form) and an exact integer form
 
 How each function is pinned (checked by tests/test_oracle.py):
@@ -65,7 +66,8 @@ def _lib():
                                os.path.join(HERE, "liboracle.so")])
     lib = ctypes.CDLL(path)
     d = ctypes.POINTER(ctypes.c_double)
-    lib.orc_assign.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, d, d, d]
+    lib.orc_assign.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, d, d, d,
+                               ctypes.c_int]
     lib.orc_measure_pk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_float, ctypes.c_int, d, d, d, d]
     lib.orc_triple_sums.argtypes = [ctypes.c_int64, d, ctypes.c_int,
@@ -143,21 +145,23 @@ def derived_params(params):
 # assignment (powerI4.f90:68-249)
 
 
-def assign_single_grid(grid, nintp, npart, pos, weight=None):
+def assign_single_grid(grid, nintp, npart, pos, weight=None, workers=None):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     assert pos.shape == (3, npart)
     w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
     out = np.zeros(grid ** 3, dtype=np.float64)
-    lib().orc_assign(grid, nintp, 0, npart, _dp(pos), None if w is None else _dp(w), _dp(out))
+    lib().orc_assign(grid, nintp, 0, npart, _dp(pos), None if w is None else _dp(w), _dp(out),
+                     workers or nthreads())
     return out.reshape((grid, grid, grid), order="F")
 
 
-def assign_double_grid(grid, nintp, npart, pos, weight=None):
+def assign_double_grid(grid, nintp, npart, pos, weight=None, workers=None):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     assert pos.shape == (3, npart)
     w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
     out = np.zeros(2 * grid ** 3, dtype=np.float64)
-    lib().orc_assign(grid, nintp, 1, npart, _dp(pos), None if w is None else _dp(w), _dp(out))
+    lib().orc_assign(grid, nintp, 1, npart, _dp(pos), None if w is None else _dp(w), _dp(out),
+                     workers or nthreads())
     return out.view(np.complex128).reshape((grid, grid, grid), order="F")
 
 
@@ -434,23 +438,38 @@ def measure_pk_multipoles(grid, box, kbin, kcenter, nbins, dcl, dcl2, dcl4):
     return pout.reshape((5, nbins), order="F")
 
 
-def pk_bin_index(grid, box, kbin, kcenter):
+def pk_bin_index(grid, box, kbin, kcenter, x0=0, x1=None):
     """imk of measure_pk for every half-space mode (ix, iy, iz) 0-based, 0 = not selected
-    (powerI4.f90:548-569).  Used to check mode counts against ps.dat."""
+    (powerI4.f90:548-569).  Used to check mode counts against ps.dat.  x0, x1: only the planes
+    x0 <= ix < x1 (default all N/2 + 1), for bounded memory at large N."""
     N = grid
     kF = np.float64(np.float32(TWOPI / np.float32(box)))
     f = _signed_freqs(N)
     h = N // 2 + 1
-    rkx = (kF * f[:h])[:, None, None]
+    x1 = h if x1 is None else min(int(x1), h)
+    fxs = f[x0:x1]
+    rkx = (kF * fxs)[:, None, None]
     rky = (kF * f)[None, :, None]
     rkz = (kF * f)[None, None, :]
     rk = np.sqrt(rkx ** 2 + rky ** 2 + rkz ** 2)
     dk = np.float64(np.float32(kbin)) * kF
     kcb = np.float64(np.float32(kcenter) / np.float32(kbin))
     imk = fnint(rk / dk - kcb - TINY) + 1
-    fx, fy, fz = f[:h][:, None, None], f[None, :, None], f[None, None, :]
+    fx, fy, fz = fxs[:, None, None], f[None, :, None], f[None, None, :]
     sel = (fx != 0) | ((fx == 0) & (fy > 0)) | ((fx == 0) & (fy == 0) & (fz > 0))
     return np.where(sel, imk, 0)
+
+
+def pk_mode_counts(grid, box, kbin, kcenter, nbins, planes=32):
+    """Mode count per P(k) bin (the co column of measure_pk), as a bincount of pk_bin_index
+    taken over blocks of x-planes."""
+    h = grid // 2 + 1
+    co = np.zeros(nbins + 2, dtype=np.int64)
+    for x0 in range(0, h, planes):
+        imk = pk_bin_index(grid, box, kbin, kcenter, x0, x0 + planes)
+        imk = np.where((imk >= 1) & (imk <= nbins), imk, 0)
+        co += np.bincount(imk.ravel(), minlength=nbins + 2)[:nbins + 2]
+    return co[1:nbins + 1]
 
 
 # --------------------------------------------------------------------------------------------
@@ -493,17 +512,19 @@ def full_spectrum(dcl):
     return F
 
 
-def shell_maps(field, grid, nbins, kbin, kcenter, workers=None):
+def shell_maps(field, grid, nbins, kbin, kcenter, workers=None, shells=None):
     """Per-shell real maps M_s(x) = Re FFTW_FORWARD(masked spectrum) (bispectrumI4.f90:142-157),
-    returned as (nbins, N^3) FP64, with nmodes per shell."""
+    returned as (nbins, N^3) FP64, with nmodes per shell.  shells: optional 1-based list; then
+    only those maps are built, in that order, with their nmodes."""
     idist = shell_index_grid(grid, kbin, kcenter)
     F = full_spectrum(field)
-    maps = np.empty((nbins, grid ** 3), dtype=np.float64)
-    nmodes = np.zeros(nbins, dtype=np.int64)
-    for s in range(1, nbins + 1):
+    which = list(range(1, nbins + 1)) if shells is None else [int(s) for s in shells]
+    maps = np.empty((len(which), grid ** 3), dtype=np.float64)
+    nmodes = np.zeros(len(which), dtype=np.int64)
+    for k, s in enumerate(which):
         m = idist == s
-        nmodes[s - 1] = int(m.sum())
-        maps[s - 1] = scipy.fft.fftn(np.where(m, F, 0.0), workers=workers or nthreads()).real.ravel()
+        nmodes[k] = int(m.sum())
+        maps[k] = scipy.fft.fftn(np.where(m, F, 0.0), workers=workers or nthreads()).real.ravel()
     return maps, nmodes
 
 
@@ -567,6 +588,51 @@ def measure_bk(nbins, grid, kbin, kcenter, iopen, kf, itot, npart, counts, dcl,
         s4 = triple_sums(maps, m4, idx, workers)
         bout[9] = 5 * kf3 * s2 / counts
         bout[10] = 9 * kf3 * s4 / counts
+    return bout
+
+
+def measure_bk_subset(nbins, grid, kbin, kcenter, iopen, kf, itot, npart, counts, dcl, rows_m,
+                      dcl2=None, dcl4=None, workers=None):
+    """Columns rows_m (0-based triangle indices of the reference loop) of measure_bk /
+    measure_bk_multipoles, with shell maps built only for the shells those triangles use.  Each
+    column is computed exactly as measure_bk computes it (same maps, same per-triangle sums, same
+    row formulas), so it equals measure_bk(...)[:, rows_m] bit for bit; counts has all itot."""
+    counts = np.asarray(counts, dtype=np.float32).astype(np.float64)
+    tl, idx = _triangle_map_index(nbins, kbin, kcenter, iopen)
+    assert len(tl) == itot
+    rows_m = np.asarray(rows_m, dtype=np.int64)
+    used = sorted(set(idx[rows_m].ravel().tolist()))           # 0-based shells
+    pos_of = {s: k for k, s in enumerate(used)}
+    sub = np.ascontiguousarray(np.vectorize(pos_of.get)(idx[rows_m]).reshape(-1, 3), dtype=np.int32)
+    kb = np.float64(np.float32(kbin))
+    kc = np.float64(np.float32(kcenter))
+    tk = kc + np.arange(nbins, dtype=np.float64) * kb
+    kf3 = kf3_bk(kf)
+    powSN = 1.0 / np.float64(npart) / kf3
+    powSN2 = powSN * powSN
+    gridD = np.float64(grid) ** 3
+    maps, nmodes = shell_maps(dcl, grid, nbins, kbin, kcenter, workers, [s + 1 for s in used])
+    pw = kf3 * np.sum(maps * maps, axis=1) / gridD / nmodes.astype(np.float64) - powSN
+    s0 = triple_sums(maps, maps, sub, workers)
+    cm = counts[rows_m]
+    rows = 9 if dcl2 is None else 11
+    bout = np.zeros((rows, len(rows_m)))
+    i, j, l = idx[rows_m, 0], idx[rows_m, 1], idx[rows_m, 2]
+    si, sj, sl = sub[:, 0], sub[:, 1], sub[:, 2]
+    bispSN = ((pw[si] + pw[sj]) + pw[sl]) * powSN + powSN2
+    bout[0], bout[1], bout[2] = tk[l], tk[j], tk[i]
+    bout[3], bout[4], bout[5] = pw[sl], pw[sj], pw[si]
+    bout[6] = kf3 * s0 / cm + bispSN
+    bout[7] = bispSN
+    bout[8] = (cm / gridD).astype(np.float32).astype(np.float64)
+    if dcl2 is not None:
+        m2, _ = shell_maps(dcl2, grid, nbins, kbin, kcenter, workers, [s + 1 for s in used])
+        s2 = triple_sums(maps, m2, sub, workers)
+        del m2
+        m4, _ = shell_maps(dcl4, grid, nbins, kbin, kcenter, workers, [s + 1 for s in used])
+        s4 = triple_sums(maps, m4, sub, workers)
+        bout[9] = 5 * kf3 * s2 / cm
+        bout[10] = 9 * kf3 * s4 / cm
     return bout
 
 

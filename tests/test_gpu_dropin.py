@@ -1,7 +1,6 @@
 """GPU tests of the drop-in layer: the reference's script call sequence, the f2py-named
 functions, the reference-layout array views, the fused P+B entry point and the serving loop
 (each result the same alone and in a mixed batch).  Tolerances as in test_gpu_parity."""
-import ctypes
 import json
 import os
 
@@ -13,7 +12,6 @@ torch = pytest.importorskip("torch")
 from oracle import pbi4_oracle as O  # noqa: E402
 
 pytestmark = pytest.mark.gpu
-REF_SO = os.path.join(os.path.dirname(O.__file__), "_ref", "libpowerI4_ref.so")
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -47,7 +45,15 @@ def _pk_close(mine, ref, rows):
         assert np.all(np.abs(mine[r][ok] - ref[r][ok]) <= 1e-4 * np.abs(ref[1][ok])), r
 
 
+def _ulp32(x):
+    """Spacing of float32 at |x|: the resolution of a value stored in the reference's float32
+    output records (pbi4/pbi4.py:482-534)."""
+    return np.spacing(np.abs(np.asarray(x, dtype=np.float32))).astype(np.float64)
+
+
 def _bk_close(mine, ref, rtol=1e-4):
+    """Both sides are float32 records.  B0 is read back as (B0 + BSN) - BSN, so besides rtol of
+    the bin's scale the difference may hold the float32 rounding of the two stored values."""
     gauss = np.sqrt(np.abs(ref[3] * ref[4] * ref[5]) / np.maximum(ref[8], 1))
     b0 = np.abs(ref[6] - ref[7])
     for r in (0, 1, 2, 8):
@@ -57,7 +63,8 @@ def _bk_close(mine, ref, rtol=1e-4):
     for r in [6] + ([9, 10] if ref.shape[0] == 11 else []):
         bref = ref[r] - (ref[7] if r == 6 else 0)
         bm = mine[r] - (mine[7] if r == 6 else 0)
-        err = np.abs(bm - bref) / np.maximum(np.maximum(np.abs(bref), gauss), b0)
+        store = _ulp32(ref[r]) + (_ulp32(ref[7]) if r == 6 else 0)
+        err = np.maximum(np.abs(bm - bref) - store, 0) / np.maximum(np.maximum(np.abs(bref), gauss), b0)
         assert err.max() <= rtol, (r, err.max())
 
 
@@ -122,7 +129,7 @@ def test_script_sequence(tmp_path, multipoles, precision):
         _pk_close(mine_p, pref.astype(np.float32).astype(np.float64), [0, 1, 2])
         names = ["k1", "k2", "k3", "Pk1", "Pk2", "Pk3", "B0+BSN", "BSN", "Ntr"]
     mine_b = np.array([bs[n] for n in names], dtype=np.float64)
-    _bk_close(mine_b, bref.astype(np.float32).astype(np.float64), rtol=2e-4)
+    _bk_close(mine_b, bref.astype(np.float32).astype(np.float64))
     # the cached counts file is read back on the next call (pbi4/pbi4.py:466-470)
     assert np.array_equal(ms.triangle_counts(), counts)
 
@@ -159,7 +166,7 @@ def test_cross_bispectrum_measurements(tmp_path):
     ref = O.measure_bk_cross(nb, N, 1.0, 1.0, True, np.float32(params["kf"]), itc, n, counts, oa, ob)
     mine = np.array([bs[c] for c in ("k1", "k2", "k3", "B0", "Ntr")], dtype=np.float64)
     _bk_cross_close(mine, ref.astype(np.float32).astype(np.float64), oa, ob, N, nb,
-                    np.float32(params["kf"]), rtol=2e-4)
+                    np.float32(params["kf"]))
 
 
 def test_cross_power_spectrum():
@@ -215,19 +222,14 @@ def test_f2py_power_functions():
 
 @pytest.mark.parametrize("L,G", [(16, 8), (12, 12), (16, 10), (9, 5), (64, 24)])
 def test_reduce_array_bit_exact(L, G):
-    # the device copy in FP64 against the oracle restatement (pinned to the reference's own
-    # reduce_array) and, where oracle/_ref travels, the reference routine itself
+    # the device copy in FP64 against the oracle restatement (pinned on the CPU to the
+    # reference's own reduce_array)
     import pbi4_amd.powerI4 as P
     rng = np.random.default_rng(3 + L + G)
     din = np.asfortranarray(rng.standard_normal((L, L, L)) + 1j * rng.standard_normal((L, L, L)))
     mine = P.reduce_array(L, din, G)
     assert mine.shape == (G, G, G) and mine.dtype == np.complex128
     assert np.array_equal(mine, O.reduce_array(L, din, G))
-    if os.path.exists(REF_SO):
-        ref = ctypes.CDLL(REF_SO)
-        out = np.zeros((G, G, G), dtype=np.complex128, order="F")
-        ref.ref_reduce_array(L, din.ctypes.data_as(ctypes.c_void_p), G, out.ctypes.data_as(ctypes.c_void_p))
-        assert np.array_equal(mine, out)
 
 
 @pytest.mark.parametrize("pdt,vdt,rsd,obj,box", [
@@ -366,7 +368,9 @@ def _bk_cross_close(mine, ref, da, db, N, nb, kf, rtol=1e-4):
         pw.append(kf3 * np.sum(maps * maps, axis=1) / float(N) ** 3 / nmodes)
     l, j, i = (ref[0] - 1).astype(int), (ref[1] - 1).astype(int), (ref[2] - 1).astype(int)
     gauss = np.sqrt(np.abs(pw[0][j] * pw[0][l] * pw[1][i]) / np.maximum(ref[4], 1))
-    err = np.abs(mine[3] - ref[3]) / np.maximum(np.abs(ref[3]), gauss)
+    # both are float32 records: allow their storage rounding on top of rtol
+    store = _ulp32(ref[3]) + _ulp32(mine[3])
+    err = np.maximum(np.abs(mine[3] - ref[3]) - store, 0) / np.maximum(np.abs(ref[3]), gauss)
     assert err.max() <= rtol, (err.max(), np.argmax(err))
 
 

@@ -380,6 +380,32 @@ def test_bispectrum(ctx, prec, multipoles, maps_grid):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("multipoles", [False, True])
+def test_bispectrum_aliased_closure(ctx, prec, multipoles):
+    """8 shells of dk = k_F on 24^3 reach |q| = 8.5, so 3 R >= N: the reference's mod-N closure
+    counts wrapped triangles (bispectrumI4.f90:884-908), the maps stay on the full grid and the
+    sums must include the wrapped terms."""
+    N, n, box, nb, kb, kc = 24, 50000, 1000.0, 8, 1.0, 1.0
+    pos = _pos(n, 91)
+    delta = ctx.density(ctx.assign(_dev(pos), N, 4, True), N, 4, True, box, n)
+    itot = O.itot_count(nb, kb, kc, True)
+    ntr = ctx.count_triangles(nb, N, kb, kc, True, False, itot)
+    assert np.array_equal(ntr, O.count_triangles_exact(nb, N, kb, kc, True))
+    # some triangles exist only through the wrap: the exact closure on a grid above 3 R has fewer
+    assert np.any(ntr > O.count_triangles_exact(nb, 64, kb, kc, True))
+    counts = (ntr.astype(np.float64) * N ** 3).astype(np.float32)
+    kf = np.float32(2 * np.pi / box)
+    los = (0.0, 0.0, 1.0)
+    desc = ctx.bk_desc(nb, N, kb, kc, True, kf, itot, n, multipoles, los, prec, 0)
+    assert ctx.maps_grid(desc) == N
+    bout = ctx.bk(delta, _dev(counts), desc).cpu().numpy().T
+    _, dcl = _oracle_density(pos, N, 4, True, box)
+    extra = O.compute_density_multipoles(dcl, N, box, np.array(los)) if multipoles else ()
+    ref = O.measure_bk(nb, N, kb, kc, True, kf, itot, n, counts, dcl, *extra)
+    _bk_close(bout, ref, 11 if multipoles else 9)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_bispectrum_explicit_multipole_fields_fused_vs_full_grid(ctx, prec):
     """Explicit delta_2 / delta_4 fields (here the densities of two other catalogues) through the
     fused z filter on the band-limited grid agree with the shell filter + 3D c2r on the full grid."""

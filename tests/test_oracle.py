@@ -156,6 +156,20 @@ def test_assign_against_reference(order):
         assert np.allclose(mine, out, rtol=1e-13, atol=1e-13), (order, inter)
 
 
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_assign_threads_bit_identical_to_serial(order):
+    # the z-slab split keeps every cell's additions in catalogue order: any thread count gives
+    # the serial loop's result bit for bit (odd N, more threads than planes, edge positions)
+    n = 40000
+    pos = _rand_pos(n, 30 + order).astype(np.float64)
+    pos[:, :4] = np.array([0.0, 1.0, 0.5, 1.0 - 2.0 ** -40])[None, :]
+    for N in (16, 7, 3):
+        for fn in (O.assign_double_grid, O.assign_single_grid):
+            serial = fn(N, order, n, pos, workers=1)
+            for t in (2, 5, 8):
+                assert np.array_equal(fn(N, order, n, pos, workers=t), serial), (N, t)
+
+
 @pytest.mark.parametrize("order", [1, 3, 4])
 def test_assign_single_particle_closed_form(order):
     N = 16
@@ -361,6 +375,14 @@ def test_pk_bin_index_matches_measure_pk_counts():
         co = O.measure_pk(N, 1000.0, kb, kc, 10, d, d)[2]
         exp = np.bincount(imk.ravel(), minlength=12)[1:11].astype(np.float64)
         assert np.array_equal(co, exp)
+        # the blocked form (planes of x) used at large N gives the same counts
+        for planes in (1, 3, 32):
+            assert np.array_equal(O.pk_mode_counts(N, 1000.0, kb, kc, 10, planes), exp)
+
+
+def test_pk_mode_counts_golden(golden_dir):
+    _, nmodes, _ = _load_ps(golden_dir)
+    assert np.array_equal(O.pk_mode_counts(64, 1000.0, 1.0, 1.0, len(nmodes), planes=5), nmodes)
 
 
 # ------------------------------------------------------------------ bispectrum
@@ -481,6 +503,71 @@ def test_pow_parseval_and_pk_invariant():
     for m, (l, j, i) in enumerate(tl):
         pw[l - 1], pw[j - 1], pw[i - 1] = bout[3, m], bout[4, m], bout[5, m]
     assert np.allclose(pw, pk[1] - psn, rtol=5e-6)
+
+
+def test_bk_multipoles_plane_waves_closed_form():
+    # delta holds waves at q1 (shell 2) and q2 (shell 3) only; delta_2 and delta_4 hold waves of
+    # different amplitudes at q3 = -(q1 + q2) (shell 4) only.  The reference attaches the L_2 / L_4
+    # map to the largest bin l and scales by 5 and 9 (bispectrumI4.f90:484-488, 499-500), so B2 and
+    # B4 are nonzero on the triangle (4, 3, 2) alone and equal the brute-force mode sums there.
+    # The ell map on bin i or j, a wrong factor, or delta_2 and delta_4 swapped all fail this.
+    N, nb, kb, kc = 16, 5, 1.0, 1.0
+    q1, q2 = np.array([2, 0, 0]), np.array([0, 3, 0])
+    q3 = -(q1 + q2)
+    assert [O.shell_of_d2(q @ q, kb, kc) for q in (q1, q2, q3)] == [2, 3, 4]
+    a = [0.7 + 0.2j, -0.3 + 0.9j]
+    b = {2: 1.1 - 0.4j, 4: -0.6 + 0.5j}
+    dcl = _plane_wave_field(N, [q1, q2], a)
+    d2 = _plane_wave_field(N, [q3], [b[2]])
+    d4 = _plane_wave_field(N, [q3], [b[4]])
+    tl = O.triangle_list(nb, kb, kc, True)
+    counts = O.count_triangles_exact(nb, N, kb, kc, True).astype(np.float64) * N ** 3
+    kf = np.float32(2 * np.pi / 1000.0)
+    bout = O.measure_bk(nb, N, kb, kc, True, kf, len(tl), 10 ** 18, counts, dcl, d2, d4)
+    kf3 = O.kf3_bk(kf)
+    va = [(q1, a[0]), (q2, a[1])]
+    va = va + [(-v, np.conj(c)) for v, c in va]
+    hits = 0
+    for row, ell, fac in ((9, 2, 5.0), (10, 4, 9.0)):
+        vl = [(q3, b[ell]), (-q3, np.conj(b[ell]))]
+        for m, (l, j, i) in enumerate(tl):
+            s = 0.0
+            for vi, ci in va:
+                for vj, cj in va:
+                    for vk, ck in vl:
+                        if np.any(vi + vj + vk):
+                            continue
+                        shells = tuple(O.shell_of_d2(v @ v, kb, kc) for v in (vi, vj, vk))
+                        if shells == (i, j, l):
+                            s += (ci * cj * ck).real
+            exp = fac * kf3 * s * N ** 3 / counts[m]
+            hits += s != 0.0
+            assert np.isclose(bout[row, m], exp, rtol=1e-10,
+                              atol=1e-12 * abs(kf3 * N ** 3 / counts[m])), (row, l, j, i)
+    assert hits == 2                                         # (4, 3, 2) in B2 and in B4
+    m = [tuple(t) for t in tl].index((4, 3, 2))
+    # the two waves of delta pair once each way round with the q3 wave: 2 Re(a1 a2 b_ell)
+    for row, ell, fac in ((9, 2, 5.0), (10, 4, 9.0)):
+        exp = fac * kf3 * 2 * (a[0] * a[1] * b[ell]).real * N ** 3 / counts[m]
+        assert np.isclose(bout[row, m], exp, rtol=1e-10)
+
+
+@pytest.mark.parametrize("multipoles", [False, True])
+def test_bk_subset_equals_full_columns(multipoles):
+    # the subset form builds maps of the sampled triangles' shells only; every column must be
+    # the full measure_bk column bit for bit (same maps, sums and row formulas)
+    N, npart, nb = 16, 8000, 6
+    pos = _rand_pos(npart, 41, clustered=True)
+    dcl = O.compute_density(npart, O.assign_double_grid(N, 4, npart, pos), N, 4, 1, 1000.0)
+    tl = O.triangle_list(nb, 1.0, 1.0, True)
+    counts = O.count_triangles_exact(nb, N, 1.0, 1.0, True).astype(np.float64) * N ** 3
+    kf = np.float32(2 * np.pi / 1000.0)
+    extra = O.compute_density_multipoles(dcl, N, 1000.0, [0, 0, 1.0]) if multipoles else (None, None)
+    full = O.measure_bk(nb, N, 1.0, 1.0, True, kf, len(tl), npart, counts, dcl, *extra)
+    for rows_m in ([0], [len(tl) - 1, 3, 17], list(range(0, len(tl), 5))):
+        sub = O.measure_bk_subset(nb, N, 1.0, 1.0, True, kf, len(tl), npart, counts, dcl, rows_m,
+                                  *extra)
+        assert np.array_equal(sub, full[:, rows_m]), rows_m
 
 
 def test_bk_multipoles_real_space_limit():
