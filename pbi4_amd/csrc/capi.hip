@@ -286,18 +286,51 @@ int maps_ld(int Np, int grid, const ShellInfo &sh) {
     return maps_fused(Np, grid, sh) ? (h + 63) / 64 * 64 : h;
 }
 
+// Full-grid maps: one 3D c2r execution transforms at most this many real outputs (all its
+// batched maps together), so no execution spans more than 2^31 elements; a 384^3 request with
+// 64 shells x 3 multipoles (1.09e10 outputs, 43 GB) runs as six executions of 32 maps.
+constexpr int64_t kMaxC2rElems = (int64_t)1 << 31;
+
+int maps_batch(int Np, int nmaps) {
+    const int64_t per = (int64_t)Np * Np * Np;
+    const int64_t cap = std::max<int64_t>(1, kMaxC2rElems / per);
+    const int64_t runs = ceil_div((int64_t)nmaps, cap);
+    return (int)ceil_div((int64_t)nmaps, runs);
+}
+
 FftPlan &maps_plan(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
                    int64_t Kstride) {
-    if (!maps_fused(Np, grid, sh)) return get_plan(ctx, 1, Np, nmaps, Kstride);
+    if (!maps_fused(Np, grid, sh)) return get_plan(ctx, 1, Np, maps_batch(Np, nmaps), Kstride);
     const bool one = Kstride == (int64_t)Np * Np * Np;
     return get_plan(ctx, 3, Np, one ? nmaps * Np : Np, maps_ld(Np, grid, sh));
+}
+
+// Work buffer bytes the maps' c2r executions need (the full-grid path may run a shorter last
+// batch with a plan of its own).
+size_t maps_work(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
+                 int64_t Kstride) {
+    size_t w = maps_plan(ctx, sh, Np, grid, nmaps, Kstride).work;
+    if (!maps_fused(Np, grid, sh)) {
+        const int b = maps_batch(Np, nmaps);
+        if (nmaps % b) w = std::max(w, get_plan(ctx, 1, Np, nmaps % b, Kstride).work);
+    }
+    return w;
 }
 
 void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, int64_t Kstride,
               float2 *spectra, float *maps, void *work, hipStream_t s) {
     FftPlan &p = maps_plan(ctx, sh, Np, grid, nmaps, Kstride);
     TimedScope ts(&ctx->timer, "pbi4_c2r", s);
-    if (!maps_fused(Np, grid, sh) || Kstride == (int64_t)Np * Np * Np) {
+    if (!maps_fused(Np, grid, sh)) {
+        const int b = maps_batch(Np, nmaps);
+        const size_t spec = (size_t)Np * Np * (Np / 2 + 1);
+        for (int m0 = 0; m0 < nmaps; m0 += b) {
+            FftPlan &q = m0 + b <= nmaps ? p : get_plan(ctx, 1, Np, nmaps - m0, Kstride);
+            fft_exec(q, spectra + (size_t)m0 * spec, maps + (size_t)m0 * Kstride, work, s);
+        }
+        return;
+    }
+    if (Kstride == (int64_t)Np * Np * Np) {
         fft_exec(p, spectra, maps, work, s);
         return;
     }
@@ -504,13 +537,13 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a, int nbins_pk 
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, L.nell, L.K, L.Kstride, d.precision);
     BinPlan b = make_bin_plan(d.grid, nbins_pk, d.nbins, 0, d.los, 1.0f);
-    FftPlan &p = maps_plan(ctx, sh, L.Np, d.grid, L.nmaps, L.Kstride);
+    const size_t work = maps_work(ctx, sh, L.Np, d.grid, L.nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(d.nbins);
     L.scale_exp = a.take<float>(d.nbins);
     L.spectra = a.take<float2>((size_t)L.nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh));
     L.maps = a.take<float>((size_t)L.nmaps * L.Kstride);
-    L.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
+    L.fftwork = a.take<char>(std::max<size_t>(work, 1));
     L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
     L.C = a.take<double>((size_t)L.g.npair * L.g.ncol);
     return L;
@@ -589,13 +622,13 @@ BkCrossLayout bk_cross_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, 2, L.K, L.Kstride, d.precision);
     BinPlan b = make_bin_plan(d.grid, 0, d.nbins, 0, d.los, 1.0f);
-    FftPlan &p = maps_plan(ctx, sh, L.Np, d.grid, nmaps, L.Kstride);
+    const size_t work = maps_work(ctx, sh, L.Np, d.grid, nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(nmaps);
     L.scale_exp = a.take<float>(nmaps);
     L.spectra = a.take<float2>((size_t)nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh));
     L.maps = a.take<float>((size_t)nmaps * L.Kstride);
-    L.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
+    L.fftwork = a.take<char>(std::max<size_t>(work, 1));
     L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
     L.C = a.take<double>((size_t)L.g.npair * L.g.ncol);
     return L;
