@@ -235,6 +235,22 @@ size_t pbi4_triangle_sums_workspace(int nb, int nell, int64_t K, int64_t Kstride
 int pbi4_triangle_sums(pbi4_ctx *ctx, int nb, int nell, int64_t K, int64_t Kstride, int precision,
                        const float *maps, double *C, void *ws, size_t ws_bytes, void *stream);
 
+/* ------------------------------------------------------------------ triangle sums, no allocation
+ * No counterpart in the reference.  pbi4_triangle_sums for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * same arguments without the workspace, same plan and kernels, so C is bitwise the same as
+ * pbi4_triangle_sums gives.  The split-K partials live in one static device array of
+ * PBI4_NOALLOC_SCRATCH_BYTES per GPU, held from library load until the process ends; shapes
+ * whose partials need more (pbi4_triangle_sums_workspace(...) - 256 bytes above the limit) are
+ * rejected with PBI4_ENOMEM.  Every call on a GPU shares that GPU's array: calls that overlap in
+ * time on one GPU (two streams, two threads, two contexts) race and must be serialised by the
+ * caller.  The context must have been created on the device (pbi4_ctx_create records the
+ * device's array address).  Nothing is allocated or freed by this call.
+ */
+#define PBI4_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pbi4_triangle_sums_noalloc(pbi4_ctx *ctx, int nb, int nell, int64_t K, int64_t Kstride,
+                               int precision, const float *maps, double *C, void *stream);
+
 /* ------------------------------------------------------------------ introspection
  * Kernel timing support for the benchmark: records HIP events around the named kernel
  * (0 = none) on every launch and returns the accumulated milliseconds and launch count.

@@ -17,6 +17,7 @@ PREC_BF16X3 = 1
 RSD_NONE, RSD_HZ, RSD_VEL = 0, 1, 2
 POS_NORMALISED = 0
 POS_BOX = 1
+NOALLOC_SCRATCH_BYTES = 64 << 20      # PBI4_NOALLOC_SCRATCH_BYTES
 
 c_int, c_float, c_double, c_int64, c_size_t, c_void_p = (ctypes.c_int, ctypes.c_float,
                                                          ctypes.c_double, ctypes.c_int64,
@@ -87,6 +88,8 @@ SIGNATURES = {
     "pbi4_triangle_sums_workspace": (c_size_t, [c_int, c_int, c_int64, c_int64, c_int]),
     "pbi4_triangle_sums": (c_int, [c_void_p, c_int, c_int, c_int64, c_int64, c_int, c_void_p,
                                    c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pbi4_triangle_sums_noalloc": (c_int, [c_void_p, c_int, c_int, c_int64, c_int64, c_int,
+                                           c_void_p, c_void_p, c_void_p]),
     "pbi4_timing_enable": (c_int, [c_void_p, ctypes.c_char_p]),
     "pbi4_timing_read": (c_int, [c_void_p, P(c_double), P(c_int64)]),
 }

@@ -100,6 +100,37 @@ struct CountsCache {
 
 }  // namespace
 
+// ------------------------------------------------------------------ allocation-free scratch
+// Split-K partials of pbi4_triangle_sums_noalloc: one static device array per GPU, held until
+// the process ends.  Host code never uses the symbol's name as a device address: each device's
+// address comes from hipGetSymbolAddress (taken once per device, with that device current, when
+// a context is created on it) and only that address reaches kernels.
+static __device__ __align__(256) unsigned char scratch[PBI4_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;   // device -> address of its `scratch`
+
+// With `device` current: records the device's scratch address (idempotent).
+void scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return;
+    void *p = nullptr;
+    PBI4_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PBI4_CHECK(p != nullptr, PBI4_EHIP, "hipGetSymbolAddress gave no address for the scratch array");
+    g_scratch_ptr[device] = p;
+}
+
+void *scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    PBI4_CHECK(it != g_scratch_ptr.end(), PBI4_EINVAL, "no scratch address for this device");
+    return it->second;
+}
+
+}  // namespace
+
 struct pbi4_ctx {
     int device = 0;
     std::string err;
@@ -649,6 +680,7 @@ int pbi4_ctx_create(int device, pbi4_ctx **out) {
         int n = 0;
         PBI4_HIP(hipGetDeviceCount(&n));
         PBI4_CHECK(device >= 0 && device < n, PBI4_EINVAL, "no such device");
+        scratch_setup(device);
     });
     if (rc != PBI4_OK) {
         delete c;
@@ -1085,6 +1117,30 @@ int pbi4_triangle_sums(pbi4_ctx *ctx, int nb, int nell, int64_t K, int64_t Kstri
         Arena a(ws, ws_bytes);
         float *part = a.take<float>(gemm_partials_floats(gp));
         launch_gemm(gp, maps, part, (hipStream_t)stream, &ctx->timer);
+        launch_gemm_reduce(gp, part, C, (hipStream_t)stream);
+    });
+}
+
+int pbi4_triangle_sums_noalloc(pbi4_ctx *ctx, int nb, int nell, int64_t K, int64_t Kstride,
+                               int precision, const float *maps, double *C, void *stream) {
+    return guarded(ctx, [&] {
+        PBI4_CHECK(maps && C, PBI4_EINVAL, "NULL argument");
+        PBI4_CHECK(nb >= 1 && (nell == 1 || nell == 3), PBI4_EINVAL, "bad nb / nell");
+        PBI4_CHECK(Kstride >= K && K > 0, PBI4_EINVAL, "bad K / Kstride");
+        PBI4_CHECK(precision == PBI4_PREC_FP32 || precision == PBI4_PREC_BF16X3, PBI4_EINVAL,
+                   "bad precision");
+        std::lock_guard<std::mutex> g(ctx->mu);
+        GemmPlan gp = make_gemm_plan(nb, nell, K, Kstride, precision);
+        const size_t need = gemm_partials_floats(gp) * sizeof(float);
+        PBI4_CHECK(need <= (size_t)PBI4_NOALLOC_SCRATCH_BYTES, PBI4_ENOMEM,
+                   "triangle sums need " + std::to_string(need) +
+                       " bytes of partials, more than the allocation-free scratch holds (" +
+                       std::to_string((size_t)PBI4_NOALLOC_SCRATCH_BYTES) +
+                       "); use pbi4_triangle_sums with a workspace");
+        // the same plan and kernels as pbi4_triangle_sums: every partial the reduce reads is
+        // written by the GEMM first, so nothing in the scratch needs clearing
+        float *part = static_cast<float *>(scratch_of(ctx->device));
+        launch_gemm(gp, maps, part, (hipStream_t)stream, nullptr);
         launch_gemm_reduce(gp, part, C, (hipStream_t)stream);
     });
 }
