@@ -24,6 +24,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+import pbi4_amd  # noqa: E402,F401  (sets how rocFFT builds its kernels before torch is used)
+
 METRIC = "P+B catalogues/sec at 256³ grid (split-BF16 vs plain FP32), p99 latency, 1e-4"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
@@ -546,39 +548,87 @@ def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
     return out
 
 
+def triangle_list(nb, kbin, kcenter, iopen):
+    """(l, j, i) of the reference's triangle loop (pbi4/pbi4.py:105-117)."""
+    ishift = int(kcenter / kbin) + (1 if iopen else 0)
+    return [(l, j, i) for l in range(1, nb + 1) for j in range(1, l + 1)
+            for i in range(max(1, l - j + 1 - ishift), j + 1)]
+
+
+def gemm_blocks(nb, nell, kbin, kcenter, iopen):
+    """16 x 16 output blocks of the triangle GEMM that the masked kernel computes (per 16-row
+    fragment of pair rows, the column tiles holding a sum the finalize reads) and the number of
+    sums it reads (the useful entries)."""
+    tiles = {}
+    tl = triangle_list(nb, kbin, kcenter, iopen)
+    for l, j, i in tl:
+        p = (j - 1) * j // 2 + (i - 1)
+        for e in range(nell):
+            tiles.setdefault(p // 16, set()).add((e * nb + l - 1) // 16)
+    return sum(len(t) for t in tiles.values()), len(tl) * nell
+
+
 def gemm_bench(ctx, torch):
-    """MFMA utilisation of the split-BF16x3 triangle GEMM at the precision-config shape: 48 shells
-    (kbin = kcenter = 1 at grid 512, max |q| = 48) on the band-limited maps grid 150^3, real space
-    (48 columns) and redshift space (144 columns); random unit-variance maps."""
+    """MFMA utilisation of the triangle GEMM at the precision-config shape: 48 shells (kbin =
+    kcenter = 1 at grid 512, max |q| = 48) on the band-limited maps grid 150^3, real space (48
+    columns) and redshift space (144 columns).
+      dense:         every (pair, column) sum, through pbi4_triangle_sums on random
+                     unit-variance maps;
+      in_bispectrum: the bispectrum's own GEMM, timed inside pbi4_measure_bk on a random 512^3
+                     half spectrum (executed = the 16 x 16 blocks the kernel issues).
+    executed_tflops counts the MFMA work issued (x3 for split-BF16), useful_tflops only the sums
+    that are read; mfma_frac = executed / peak."""
     from pbi4_amd import _lib
     out = {}
-    nb, Np = 48, 150
+    nb, Np, N = 48, 150, 512
     K = Np ** 3
     Ks = ((K + 63) // 64) * 64
+    npair = nb * (nb + 1) // 2
     g = torch.Generator(device="cuda")
     g.manual_seed(7)
+    itot = len(triangle_list(nb, 1.0, 1.0, True))
+    ntr = ctx.count_triangles(nb, N, 1.0, 1.0, True, False, itot)
+    counts = torch.from_numpy((ntr.astype(np.float64) * float(N) ** 3).astype(np.float32)).cuda()
+    delta = torch.randn((N, N, N // 2 + 1), generator=g, device="cuda", dtype=torch.complex64)
+    kf = np.float32(2 * np.pi / 1000.0)
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        ctx.timing_enable("pbi4_triangle_gemm")
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        ms, n = ctx.timing_read()
+        ctx.timing_enable(None)
+        return ms / max(n, 1)
+
     for nell in (1, 3):
         maps = torch.randn((nb * nell, Ks), generator=g, device="cuda", dtype=torch.float32)
         maps[:, K:] = 0
-        npair = nb * (nb + 1) // 2
+        blocks, useful = gemm_blocks(nb, nell, 1.0, 1.0, True)
         res = {}
-        for prec, name, peak in ((_lib.PREC_BF16X3, "bf16x3", BF16_PEAK_TFLOPS),
-                                 (_lib.PREC_FP32, "fp32", 157.3)):
-            ctx.triangle_sums(maps, nb, nell, K, prec)
-            torch.cuda.synchronize()
-            ctx.timing_enable("pbi4_triangle_gemm")
-            reps = 5
-            for _ in range(reps):
-                ctx.triangle_sums(maps, nb, nell, K, prec)
-            torch.cuda.synchronize()
-            ms, n = ctx.timing_read()
-            ctx.timing_enable(None)
-            avg = ms / max(n, 1)
-            logical = 2.0 * npair * nb * nell * K
-            executed = logical * (3 if prec == _lib.PREC_BF16X3 else 1)
+        for prec, name, peak, x in ((_lib.PREC_BF16X3, "bf16x3", BF16_PEAK_TFLOPS, 3),
+                                    (_lib.PREC_FP32, "fp32", 157.3, 1)):
+            avg = timed(lambda: ctx.triangle_sums(maps, nb, nell, K, prec))
+            executed = x * 2.0 * npair * nb * nell * K
             tf = executed / (avg * 1e-3) / 1e12
-            res[name] = {"avg_ms": round(avg, 4), "executed_tflops": round(tf, 1),
-                         "mfma_frac": round(tf / peak, 4), "logical_tflops": round(logical / (avg * 1e-3) / 1e12, 1)}
+            res[name + "_dense"] = {"avg_ms": round(avg, 4), "executed_tflops": round(tf, 1),
+                                    "mfma_frac": round(tf / peak, 4),
+                                    "useful_tflops": round(x * 2.0 * useful * K / (avg * 1e-3) / 1e12, 1)}
+            desc = ctx.bk_desc(nb, N, 1.0, 1.0, True, kf, itot, 10 ** 8, nell == 3,
+                               (0.0, 0.0, 1.0), prec, 0)
+            assert ctx.maps_grid(desc) == Np
+            ws = ctx._ws(ctx.bk_workspace(desc))
+            avg = timed(lambda: ctx.bk(delta, counts, desc, ws=ws))
+            executed = x * 2.0 * npair * nb * nell * K
+            tf = executed / (avg * 1e-3) / 1e12
+            res[name + "_in_bispectrum"] = {
+                "avg_ms": round(avg, 4), "executed_tflops": round(tf, 1),
+                "mfma_frac": round(tf / peak, 4),
+                "useful_tflops": round(x * 2.0 * useful * K / (avg * 1e-3) / 1e12, 1),
+                "blocks_needed_of_dense": round(blocks / (-(-npair // 16) * -(-nb * nell // 16)), 4)}
+            del ws
         out[f"nb48_nell{nell}_K150^3"] = res
         del maps
     log(f"[bench] triangle GEMM alone: {out}")

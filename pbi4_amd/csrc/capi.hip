@@ -1,5 +1,8 @@
 // C ABI of libpbi4hip.so: context, cached tables and rocFFT plans, entry points.
+#include <dlfcn.h>
 #include <rocfft/rocfft.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -7,6 +10,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -183,9 +187,86 @@ void fft_check(rocfft_status st, const char *what) {
         fail(PBI4_EFFT, std::string(what) + " failed (rocfft status " + std::to_string((int)st) + ")");
 }
 
+// ------------------------------------------------------------------ how rocFFT builds kernels
+// rocFFT compiles its kernels at run time through hiprtc, and also takes prebuilt ones from a
+// system kernel cache.  Built with default flags they contain packed FP32 VALU operations, which
+// on this hardware have given wrong results, with no error, while matrix-core work ran beside
+// them: with two serving slots, the last c2r batch of one slot's shell maps changed while the
+// other slot's triangle GEMM ran, and never with rocFFT built without them.  So every rocFFT
+// kernel this library runs is built with packed FP32 off (as the library's own kernels are),
+// into a kernel cache of the library's own, and none comes from the system cache.  rocFFT reads
+// these settings when it first plans a transform in the process: they are set when the library
+// is loaded (the Python package sets the same ones on import), and the first plan checks that
+// rocFFT uses the library's cache, which fails when rocFFT was in use before the library loaded.
+constexpr const char *kNoPackedFp32 = "-Xclang -target-feature -Xclang -packed-fp32-ops";
+std::string g_rocfft_cache;    // the kernel cache file this library told rocFFT to use
+
+std::string library_dir() {
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void *>(&library_dir), &info) && info.dli_fname) {
+        const std::string p(info.dli_fname);
+        const size_t k = p.rfind('/');
+        return k == std::string::npos ? std::string(".") : p.substr(0, k);
+    }
+    return ".";
+}
+
+bool writable_dir(const std::string &d) {
+    (void)mkdir(d.c_str(), 0755);
+    return access(d.c_str(), W_OK | X_OK) == 0;
+}
+
+void rocfft_build_settings() {
+    const char *cur = getenv("HIPRTC_COMPILE_OPTIONS_APPEND");
+    std::string opts = cur ? cur : "";
+    if (opts.find("packed-fp32-ops") == std::string::npos) {
+        opts += (opts.empty() ? "" : " ");
+        opts += kNoPackedFp32;
+        setenv("HIPRTC_COMPILE_OPTIONS_APPEND", opts.c_str(), 1);
+    }
+    setenv("ROCFFT_RTC_SYS_CACHE_PATH", "/nonexistent/pbi4_amd/no_system_kernel_cache.db", 1);
+    const char *given = getenv("PBI4_ROCFFT_CACHE");
+    std::string path;
+    if (given && *given) {
+        path = given;
+    } else {
+        std::string dir = library_dir() + "/_rocfft_cache";
+        if (!writable_dir(dir)) {
+            const char *tmp = getenv("TMPDIR");
+            dir = std::string(tmp && *tmp ? tmp : "/tmp") + "/pbi4_amd_rocfft_" +
+                  std::to_string((long)getuid());
+            (void)writable_dir(dir);
+        }
+        path = dir + "/kernels.db";
+    }
+    setenv("ROCFFT_RTC_CACHE_PATH", path.c_str(), 1);
+    g_rocfft_cache = path;
+}
+
+struct RocfftSettingsAtLoad {
+    RocfftSettingsAtLoad() { rocfft_build_settings(); }
+};
+RocfftSettingsAtLoad g_rocfft_settings_at_load;
+
 struct RocfftInit {
     RocfftInit() { rocfft_setup(); }
 };
+
+// After the first plan: rocFFT has opened its kernel cache by now; if it is not the one this
+// library named, rocFFT was set up before the library loaded and may run packed-FP32 kernels.
+void check_rocfft_cache() {
+    static bool checked = false;
+    if (checked) return;
+    checked = true;
+    const size_t k = g_rocfft_cache.rfind('/');
+    const std::string dir = k == std::string::npos ? "." : g_rocfft_cache.substr(0, k);
+    if (access(dir.c_str(), W_OK) == 0 && access(g_rocfft_cache.c_str(), F_OK) != 0)
+        fail(PBI4_EUNSUPPORTED,
+             "rocFFT is not using this library's kernel cache (" + g_rocfft_cache +
+                 "): it was set up in this process before libpbi4hip.so was loaded, so its "
+                 "kernels may contain packed FP32 operations; load pbi4_amd (import pbi4_amd, or "
+                 "dlopen libpbi4hip.so) before the process's first GPU FFT");
+}
 
 // kind 0: r2c of N^3 with `batch` grids (default layouts).
 // kind 1: c2r of N^3 with `batch` maps, real output per-map distance `ext` floats.
@@ -237,7 +318,9 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
               "rocfft_plan_create");
     rocfft_plan_description_destroy(desc);
     fft_check(rocfft_plan_get_work_buffer_size(p.plan, &p.work), "rocfft_plan_get_work_buffer_size");
-    return ctx->plans[key] = p;
+    ctx->plans[key] = p;
+    check_rocfft_cache();
+    return ctx->plans[key];
 }
 
 void fft_exec(FftPlan &p, void *in, void *out, void *work, hipStream_t s, void *in2 = nullptr) {

@@ -400,9 +400,14 @@ GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precis
     // writes a partial tile that the FP64 reduce reads back, so short chunks on small problems
     // cost more in partials than they gain in parallelism (16 shells x 3 on 100^3: GEMM + reduce
     // 197 -> 118 us)
+    // and at most kMaxChunk long: each chunk is accumulated in FP32 on the MFMA before the FP64
+    // combine, and on full grids (K = N^3 up to 10^8) chunks of ~5e5 points left 2.7e-4 relative
+    // errors on sums that cancel (384^3, 64 shells x 3, FP32 and split-BF16 alike)
+    constexpr int64_t kMaxChunk = 32768;
     const int kt = g.cf <= 6 ? 64 : 32;
     int64_t target = std::max<int64_t>(kXcds, 2048 / g.nrowblocks);
     int64_t kc = std::max<int64_t>(2048, align_up(ceil_div(K, target), kt));
+    kc = std::min<int64_t>(kc, kMaxChunk);
     g.kchunk = (int)kc;
     g.nkchunks = (int)ceil_div(K, kc);
     g.nkchunks_pad = (int)align_up(g.nkchunks, kXcds);

@@ -453,10 +453,15 @@ def test_serving_alone_and_in_the_mix():
     m, (p, v) = jobs[0]
     res = done[0][Field(False)]
     N = m.grid
-    pos = (p.cpu().numpy() % 1000.0) / 1000.0
+    # the reference's normalisation in float32 (pbi4/pbi4.py:241-243), as the device applies it
+    box = np.float32(1000.0)
+    pos = np.minimum(((p.cpu().numpy() + box) % box) / box, np.float32(1.0))
     npart = pos.shape[1]
     d = O.compute_density(npart, O.assign_double_grid(N, 4, npart, pos.astype(np.float64)), N, 4, True, 1000.0)
     nb, itot = m.nb_bisp(), m.itot()
     rc = (O.count_triangles_exact(nb, N, 2.0, 2.0, True) * float(N) ** 3).astype(np.float32)
     bref = O.measure_bk(nb, N, 2.0, 2.0, True, m.kf(), itot, npart, rc, d)
-    _bk_close(res[1].T, bref)
+    # 3 R >= N here: the maps are on the full grid, where split-BF16 sums are held to 3e-4
+    # (the full-grid bound of tests/test_gpu_configs.py); FP32 sums to 1e-4
+    assert ctx.maps_grid(ctx.bk_desc(nb, N, 2.0, 2.0, True, m.kf(), itot, npart)) == N
+    _bk_close(res[1].T, bref, rtol=3e-4)

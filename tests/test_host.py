@@ -197,3 +197,14 @@ def test_assign_tile_edge_follows_particle_density():
                         (192, 1 << 24, 12), (92, 60000, 4), (20, 1 << 20, 10)]:
         r = rest(N, npart)
         assert regions(N, T) <= r <= regions(N, T) + 3 * 4 * ((N // T) ** 3 + 1) + 8192, (N, npart)
+
+
+def test_import_sets_rocfft_kernel_build():
+    import os
+    # rocFFT builds its kernels with packed FP32 off, into the package's own kernel cache, and
+    # takes none from the system cache (set on import, before torch is used)
+    import pbi4_amd  # noqa: F401
+    assert "-packed-fp32-ops" in os.environ["HIPRTC_COMPILE_OPTIONS_APPEND"]
+    assert os.environ["ROCFFT_RTC_SYS_CACHE_PATH"].startswith("/nonexistent/")
+    assert os.environ["ROCFFT_RTC_CACHE_PATH"] == os.environ["PBI4_ROCFFT_CACHE"]
+    assert os.path.isdir(os.path.dirname(os.environ["ROCFFT_RTC_CACHE_PATH"]))
