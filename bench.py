@@ -1,16 +1,33 @@
 """Benchmark: P+B catalogues/sec at 256^3 (split-BF16 serving path vs plain FP32), p99 latency.
 
-Workload (one "step" = one batch of `--slots` catalogues through the serving engine):
+Headline workload (one "step" = one batch of `--slots` catalogues through the serving engine):
   256^3 Zel'dovich particles (16.8M, float32 positions + velocities) in L = 1000 Mpc/h,
   grid 256^3, PCS with interlacing, real space and redshift space (LOS z); P_l (l = 0, 2, 4 in
   redshift space) with kbin = kcenter = 2; bispectrum with kbin = kcenter = 2 up to 32 k_F
   (16 shells, 564 triangles, open triangles), B_0 in real space and B_0,2,4 in redshift space.
 Inputs are resident in HBM before the timed region.  `value` = catalogues/sec of the serving
-path (split-BF16x3 triangle GEMM, band-limited maps grid, concurrent slots).  Also reported:
-the plain one-at-a-time FP32 path (reference structure: maps on the full 256^3 grid, FP32
-triangle sums, host synchronisation after every call), p50/p99 latency of a Poisson stream at
-80% of the measured capacity, the per-bin split-BF16 error, the roofline of one kernel timed with
-HIP events inside the timed region, and the FP64 CPU oracle on a bounded sample.
+path (split-BF16x3 triangle GEMM, band-limited maps grid, concurrent slots).
+
+Also reported, each as its own key of the one JSON line (stdout; progress goes to stderr):
+  speedup     the same workload one catalogue at a time with a host synchronisation after every
+              call: plain FP32 on the full maps grid (the reference's structure), plain FP32 on
+              the band-limited grid, split-BF16 on the band-limited grid; the ratios between
+              them and the serving value (band-limiting, split-BF16, slot overlap, like for like)
+  latency     p50/p99 of a Poisson stream of --latency-requests requests at 80% of the capacity
+  precision   per-bin B errors of the split-BF16 and FP32 results against the FP64 oracle and
+              against each other, for one catalogue (both fields, all 564 triangles)
+  cpu_baseline  the FP64 oracle timed in full on the host cores at this workload (and at the
+              128^3 / 64^3 TSC workload), no extrapolation
+  cfg4        a Poisson stream of --cfg4-requests requests over grids 128..384, CIC/TSC/PCS,
+              1e6-5e7 particles, with the reference's default shell count per grid, against the
+              plain FP32 path one request at a time
+  cfg3        512^3 particles, grid 512^3, 48 shells of k_F: split-BF16 vs FP32 throughput and
+              per-bin error
+  cfg5        1024^3 particles, grid 1024^3, three lines of sight, 32 shells: seconds per
+              catalogue, deposit GB/s, GEMM MFMA fraction
+  triangle_gemm, permuted_assign, stage_ms_rsd_field
+A deadline (--deadline seconds from process start) bounds the whole run: an optional leg that
+would not fit is shortened or skipped, and says so in its key.
 """
 import argparse
 import json
@@ -19,7 +36,9 @@ import os
 import sys
 import time
 
-import numpy as np
+T_START = time.perf_counter()
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -29,10 +48,11 @@ import pbi4_amd  # noqa: E402,F401  (sets how rocFFT builds its kernels before t
 METRIC = "P+B catalogues/sec at 256³ grid (split-BF16 vs plain FP32), p99 latency, 1e-4"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+BOX = 1000.0
 
 
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T_START:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
 def parse():
@@ -44,31 +64,35 @@ def parse():
     ap.add_argument("--pool", type=int, default=4, help="distinct resident catalogues")
     ap.add_argument("--lattice", type=int, default=256)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--plain", type=int, default=3, help="catalogues for the plain path (0 = skip)")
-    ap.add_argument("--latency-requests", type=int, default=24)
-    ap.add_argument("--mixed-requests", type=int, default=200,
-                    help="requests of the mixed-grid Poisson stream (0 = skip)")
-    ap.add_argument("--mixed-pool", type=int, default=24, help="distinct catalogues it cycles")
-    ap.add_argument("--mixed-shells", type=int, default=16,
-                    help="bispectrum shells of every mixed-stream request; 0 = the reference's "
-                         "default per grid (21, 32, 42, 64 at 128, 192, 256, 384)")
-    ap.add_argument("--mixed-slots", type=int, default=4, help="serving slots of the mixed stream")
-    ap.add_argument("--mixed-only", type=int, default=0,
-                    help="run only the mixed stream, print it and exit")
+    ap.add_argument("--plain", type=int, default=3,
+                    help="catalogues per one-at-a-time point of the speedup ladder (0 = skip)")
+    ap.add_argument("--latency-requests", type=int, default=1000)
     ap.add_argument("--roofline-kernel", default="pbi4_deposit")
-    ap.add_argument("--roofline-steps", type=int, default=4,
-                    help="one-slot catalogues of the timed pass that times the roofline kernel")
+    ap.add_argument("--roofline-steps", type=int, default=4)
+    ap.add_argument("--precision", type=int, default=1, help="per-bin B errors vs FP64")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--stages", type=int, default=1, help="print a per-stage breakdown")
-    ap.add_argument("--gemm", type=int, default=1,
-                    help="time the triangle GEMM alone at the 512^3 / 48-shell shape")
-    ap.add_argument("--large", type=int, default=0,
-                    help="measure the large single box with this many particles per side and grid "
-                         "(1024 for the full configuration; three lines of sight, 32 shells) "
-                         "instead of the serving workload, print it and exit")
+    ap.add_argument("--stages", type=int, default=1)
+    ap.add_argument("--cfg4-requests", type=int, default=2000)
+    ap.add_argument("--cfg4-pool", type=int, default=24)
+    ap.add_argument("--cfg4-fp32", type=int, default=100,
+                    help="requests of the cfg4 stream timed through the plain FP32 path")
+    ap.add_argument("--cfg3", type=int, default=1)
+    ap.add_argument("--cfg5", type=int, default=1024, help="particles per side and grid (0 = skip)")
+    ap.add_argument("--gemm", type=int, default=1)
+    ap.add_argument("--permuted", type=int, default=1)
+    ap.add_argument("--deadline", type=float, default=530.0,
+                    help="seconds from process start by which the run must end")
     ap.add_argument("--gemm-only", type=int, default=0,
                     help="run only the triangle GEMM timing (for profiling), print it and exit")
+    ap.add_argument("--cfg4-only", type=int, default=0,
+                    help="run only the cfg4 stream, print it and exit")
+    ap.add_argument("--large-only", type=int, default=0,
+                    help="run only the cfg5 large box with this many particles per side and exit")
     return ap.parse_args()
+
+
+def remaining(args):
+    return args.deadline - (time.perf_counter() - T_START)
 
 
 def pmc_traffic(kernel):
@@ -94,24 +118,70 @@ def sigma_b(bout_t, kf):
                    / float(kf) ** 3)
 
 
-def bf16_error(ref_out, out, kf):
-    """Max / median / p99 per-bin difference of the B rows of `out` from `ref_out` ({field:
-    (pout, bout)} host arrays), relative to max(|B_l|, |B_0|, Gaussian sigma of the bin)."""
-    worst = 0.0
-    per_bin = []
-    for f in ref_out:
-        rb, mb = ref_out[f][1].T, out[f][1].T
-        gauss = sigma_b(rb, kf)
-        b0 = np.abs(rb[6] - rb[7])
-        rows = [6] + ([9, 10] if rb.shape[0] == 11 else [])
-        for r in rows:
-            bref = rb[r] - (rb[7] if r == 6 else 0)
-            bm = mb[r] - (mb[7] if r == 6 else 0)
-            err = np.abs(bm - bref) / np.maximum(np.maximum(np.abs(bref), gauss), b0)
-            per_bin.append(err)
-            worst = max(worst, float(err.max()))
-    allb = np.concatenate(per_bin)
-    return worst, float(np.median(allb)), float(np.percentile(allb, 99))
+def b_errors(ref, mine, kf):
+    """Per-bin |B - B_ref| of the B rows (B0 from rows 6 - 7, then B2, B4) of two bout arrays
+    (rows x itot), relative to max(|B_l|, |B0|, sigma_B), sigma_B the bin's Gaussian standard
+    deviation.  Returns {row name: array}."""
+    fl = sigma_b(ref, kf)
+    b0 = np.abs(ref[6] - ref[7])
+    out = {}
+    for r, name in ((6, "B0"), (9, "B2"), (10, "B4")):
+        if r >= ref.shape[0]:
+            continue
+        br = ref[r] - (ref[7] if r == 6 else 0)
+        bm = mine[r] - (mine[7] if r == 6 else 0)
+        out[name] = np.abs(bm - br) / np.maximum(np.maximum(np.abs(br), fl), b0)
+    return out
+
+
+def tenths_of_decades(err):
+    """Compact per-bin encoding for the JSON line: one two-digit field per bin, v =
+    round(-10 log10(err)) in 00..99 (so err = 10^(-v/10); 99 also stands for anything below
+    1.3e-10), concatenated in the reference's triangle order."""
+    e = np.maximum(np.asarray(err, dtype=np.float64), 1e-300)
+    return "".join(f"{int(v):02d}" for v in np.clip(np.rint(-10.0 * np.log10(e)), 0, 99))
+
+
+def stats(err):
+    e = np.concatenate([np.ravel(x) for x in err]) if isinstance(err, (list, tuple)) else np.ravel(err)
+    return {"max": float(e.max()), "p99": float(np.percentile(e, 99)), "median": float(np.median(e)),
+            "bins": int(e.size)}
+
+
+def triangle_list(nb, kbin, kcenter, iopen):
+    """(l, j, i) of the reference's triangle loop (pbi4/pbi4.py:105-117)."""
+    ishift = int(kcenter / kbin) + (1 if iopen else 0)
+    return [(l, j, i) for l in range(1, nb + 1) for j in range(1, l + 1)
+            for i in range(max(1, l - j + 1 - ishift), j + 1)]
+
+
+def gemm_blocks(nb, nell, kbin, kcenter, iopen):
+    """16 x 16 output blocks of the triangle GEMM that hold a sum the finalize reads (per 16-row
+    fragment of pair rows, the column tiles with such a sum) and the number of sums it reads."""
+    tiles = {}
+    tl = triangle_list(nb, kbin, kcenter, iopen)
+    for l, j, i in tl:
+        p = (j - 1) * j // 2 + (i - 1)
+        for e in range(nell):
+            tiles.setdefault(p // 16, set()).add((e * nb + l - 1) // 16)
+    return sum(len(t) for t in tiles.values()), len(tl) * nell
+
+
+def guarded(name, fn, out, args, need):
+    """Run leg `fn` if at least `need` seconds remain before the deadline; record its result or
+    why it did not run under out[name]."""
+    left = remaining(args)
+    if left < need:
+        out[name] = {"skipped": f"{left:.0f} s left before the deadline, the leg needs ~{need:.0f} s"}
+        log(f"{name}: skipped ({left:.0f} s left)")
+        return
+    t0 = time.perf_counter()
+    try:
+        out[name] = fn()
+    except Exception as e:  # the other legs still report
+        out[name] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"{name}: error {e}")
+    log(f"{name}: {time.perf_counter() - t0:.1f} s")
 
 
 def main():
@@ -132,65 +202,60 @@ def main():
     if args.gemm_only:
         print(json.dumps({"triangle_gemm": gemm_bench(ctx, torch)}), flush=True)
         return 0
-    if args.large:
-        print(json.dumps(large_box(ctx, torch, n=args.large)), flush=True)
+    if args.cfg4_only:
+        print(json.dumps({"cfg4": cfg4_stream(ctx, torch, args, 1e9)}), flush=True)
         return 0
-    if args.mixed_only:
-        print(json.dumps({"mixed_stream": mixed_stream(ctx, torch, args.mixed_requests,
-                                                       args.mixed_pool, 1000.0,
-                                                       args.mixed_shells, args.mixed_slots)}),
-              flush=True)
+    if args.large_only:
+        print(json.dumps(large_box(ctx, torch, n=args.large_only)), flush=True)
         return 0
     N, n = args.grid, args.lattice
-    box = 1000.0
-    m = Measurement(grid=N, order=4, interlacing=True, box=box, kbin=2.0, kcenter=2.0,
-                    nbins_ps=0, nbins_bisp=16, iopen=True,
-                    fields=(Field(False), Field(True, (0.0, 0.0, 1.0))),
-                    precision=_lib.PREC_BF16X3, maps_grid=0)
-    log(f"[bench] generating {args.pool} catalogues of {n}^3 particles on the GPU")
-    pool = [zeldovich(n, box=box, seed=1000 + i, device="cuda") for i in range(args.pool)]
+    fields = (Field(False), Field(True, (0.0, 0.0, 1.0)))
+
+    def meas(prec, mg):
+        return Measurement(grid=N, order=4, interlacing=True, box=BOX, kbin=2.0, kcenter=2.0,
+                           nbins_ps=0, nbins_bisp=16, iopen=True, fields=fields, precision=prec,
+                           maps_grid=mg)
+
+    m = meas(_lib.PREC_BF16X3, 0)
+    log(f"generating {args.pool} catalogues of {n}^3 particles on the GPU")
+    pool = [zeldovich(n, box=BOX, seed=1000 + i, device="cuda") for i in range(args.pool)]
     torch.cuda.synchronize()
     npart = n ** 3
-    t0 = time.perf_counter()
     plan = CataloguePlan(ctx, m)
     torch.cuda.synchronize()
-    log(f"[bench] plan ready in {time.perf_counter() - t0:.2f}s: itot={plan.itot} "
-        f"maps grid {plan.maps_grid}^3")
+    log(f"plan ready: itot={plan.itot} maps grid {plan.maps_grid}^3")
 
     eng = ServingEngine(ctx, slots=args.slots)
     eng.plans[m] = plan
     rid = [0]
 
-    def batch(k):
+    def batch(k, mm=m):
         reqs = []
         for _ in range(k):
             p, v = pool[rid[0] % len(pool)]
-            reqs.append(Request(rid[0], m, p, v, arrival=0.0))
+            reqs.append(Request(rid[0], mm, p, v, arrival=0.0))
             rid[0] += 1
         return reqs
 
-    def step():
-        return eng.run(batch(args.slots))
-
     for _ in range(args.warmup):
-        step()
+        eng.run(batch(args.slots))
     torch.cuda.synchronize()
-    log("[bench] warmup done")
+    log("warmup done")
 
+    # ---------------------------------------------------------------- headline (timed region)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        eng.run(batch(args.slots))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     n_cat = args.steps * args.slots
     value = n_cat / elapsed
-    log(f"[bench] serving: {n_cat} catalogues in {elapsed:.3f}s -> {value:.2f} catalogues/s")
+    log(f"serving: {n_cat} catalogues in {elapsed:.3f}s -> {value:.2f} catalogues/s")
 
-    # ---------------------------------------------------------------- timed pass for the roofline
+    # ---------------------------------------------------------------- roofline kernel timing
     # the same workload through a one-slot engine: launches do not overlap, so the HIP event pair
-    # the library records around each launch on its stream brackets exactly one kernel (with
-    # several slots the pair also holds the time the launch queues behind other slots' kernels)
+    # the library records around each launch on its stream brackets exactly one kernel
     eng1 = ServingEngine(ctx, slots=1)
     eng1.plans[m] = plan
     eng1.run(batch(1))
@@ -203,132 +268,133 @@ def main():
     ctx.timing_enable(None)
     eng1.release()
     del eng1
-
-    # ---------------------------------------------------------------- roofline of one kernel
     avg_ms = kern_ms / max(launches, 1)
     V = N ** 3
-    ntiles = (N // 16) ** 3
-    if args.roofline_kernel == "pbi4_deposit":
-        # per launch: read the tile-sorted particles (12 B each), write both interlaced grids'
-        # halo regions once (algorithmic = the 2 x 4 B x V grid cells)
-        alg_bytes = 12.0 * npart + 8.0 * V
-        roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-        achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    elif args.roofline_kernel == "pbi4_triangle_gemm":
-        # executed bf16 MFMA flops per launch (averaged over the real-space launch with nb
-        # columns and the redshift-space launch with 3 nb columns): 3 x 2 x pairs x cols x K
-        nb = m.nb_bisp()
-        npair = nb * (nb + 1) // 2
-        K = plan.maps_grid ** 3
-        flops = 3 * 2.0 * npair * nb * (1 + 3) * K / 2
-        roof = {"bound": "mfma", "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s"}
-        achieved = flops / (avg_ms * 1e-3) / 1e12
-        alg_bytes = None
-    else:
-        alg_bytes = None
-        roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-        achieved = float("nan")
-    roof.update({"kernel": args.roofline_kernel, "achieved": round(achieved, 2),
-                 "frac": round(achieved / roof["peak"], 4),
-                 "traffic": pmc_traffic("k_deposit<4, 2, 20>") if args.roofline_kernel == "pbi4_deposit" else None,
-                 "avg_launch_ms": round(avg_ms, 5), "launches": launches,
-                 "timing": "HIP events around each launch on its stream, one-slot timed pass of "
-                           "the same workload",
-                 "algorithmic_bytes_per_launch": alg_bytes})
+    # per launch: read the tile-sorted particles (12 B each) and write both interlaced grids
+    # once (2 x 4 B per cell): the deposit's algorithmic bytes
+    alg_bytes = 12.0 * npart + 8.0 * V
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("k_deposit<4, 2, 20>"),
+            "kernel": args.roofline_kernel, "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+            "timing": "HIP events around each launch on its stream, one-slot pass of the workload",
+            "algorithmic_bytes_per_launch": alg_bytes}
+
+    out = {}
 
     # ---------------------------------------------------------------- per-stage breakdown
-    stages = {}
     if args.stages:
         s = torch.cuda.current_stream()
         p, v = pool[0]
         rsd = tuple(float(np.float32(plan.rsd_coef * l)) for l in (0.0, 0.0, 1.0))
         names = ["assign", "density", "pk", "bk"]
-        for rep in range(2):
+        for _ in range(2):
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
             ev[0].record(s)
-            g = ctx.assign(p, N, 4, True, vel=v, box=box, rsd=rsd, normalised=False)
+            g = ctx.assign(p, N, 4, True, vel=v, box=BOX, rsd=rsd, normalised=False)
             ev[1].record(s)
-            d = ctx.density(g, N, 4, True, box, npart)
+            d = ctx.density(g, N, 4, True, BOX, npart)
             ev[2].record(s)
-            ctx.pk(d, N, box, 2.0, 2.0, m.nb_ps(), True)
+            ctx.pk(d, N, BOX, 2.0, 2.0, m.nb_ps(), True)
             ev[3].record(s)
-            desc = plan.desc[Field(True, (0.0, 0.0, 1.0))]
+            desc = plan.desc[fields[1]]
             desc.npart = npart
             ctx.bk(d, plan.counts, desc)
             ev[4].record(s)
             torch.cuda.synchronize()
-        stages = {nm: round(ev[i].elapsed_time(ev[i + 1]), 4) for i, nm in enumerate(names)}
-        log(f"[bench] redshift-space field stages (ms): {stages}")
+        out["stage_ms_rsd_field"] = {nm: round(ev[i].elapsed_time(ev[i + 1]), 4)
+                                     for i, nm in enumerate(names)}
+        log(f"redshift-space field stages (ms): {out['stage_ms_rsd_field']}")
 
-    # ---------------------------------------------------------------- plain one-at-a-time FP32
-    plain_value = None
-    errs = None
+    # ---------------------------------------------------------------- speedup ladder
+    results = {}
     if args.plain > 0:
-        mp = Measurement(**{**m.__dict__, "precision": _lib.PREC_FP32, "maps_grid": N})
-        pplan = CataloguePlan(ctx, mp)
+        def one_at_a_time(prec, mg):
+            mm = meas(prec, mg)
+            pp = CataloguePlan(ctx, mm)
+            e1 = ServingEngine(ctx, slots=1)
+            e1.plans[mm] = pp
+            e1.run(batch(1, mm))                     # warm: plans, workspace
+            torch.cuda.synchronize()
+            rid[0] = 0                               # the timed catalogues start at pool[0]
+            t = time.perf_counter()
+            done = []
+            for _ in range(args.plain):
+                done += e1.run(batch(1, mm))         # returns after the host has the result
+            rate = args.plain / (time.perf_counter() - t)
+            e1.release()
+            res = {r.rid % len(pool): r.result for r in done}
+            return rate, res, pp.maps_grid
 
-        def plain_one(p, v):
-            outs = {}
-            for f in mp.fields:
-                rsd = tuple(float(np.float32(pplan.rsd_coef * l)) for l in f.los) if f.rsd else None
-                g = ctx.assign(p, N, 4, True, vel=v if f.rsd else None, box=box, rsd=rsd,
-                               normalised=False)
-                torch.cuda.synchronize()
-                d = ctx.density(g, N, 4, True, box, npart)
-                torch.cuda.synchronize()
-                po = ctx.pk(d, N, box, 2.0, 2.0, mp.nb_ps(), f.rsd, f.los).cpu().numpy()
-                desc = pplan.desc[f]
-                desc.npart = npart
-                bo = ctx.bk(d, pplan.counts, desc).cpu().numpy()
-                outs[f] = (po, bo)
-            return outs
-
-        plain_one(*pool[0])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ref_outs = []
-        for i in range(args.plain):
-            ref_outs.append(plain_one(*pool[i % len(pool)]))
-        plain_value = args.plain / (time.perf_counter() - t0)
-        log(f"[bench] plain FP32 one-at-a-time: {plain_value:.2f} catalogues/s")
-        # per-bin error of the serving (split-BF16, band-limited) result vs the plain FP32 one
-        res = eng.run(batch(1))[0].result
-        idx = (rid[0] - 1) % len(pool)
-        plain_same = ref_outs[idx] if idx < len(ref_outs) else plain_one(*pool[idx])
-        errs = bf16_error(plain_same, res, m.kf())
-        log(f"[bench] serving vs plain FP32 B error: max {errs[0]:.2e} median {errs[1]:.2e} "
-            f"p99 {errs[2]:.2e}")
+        ladder = {}
+        for key, prec, mg in (("plain_fp32_full_grid", _lib.PREC_FP32, N),
+                              ("plain_fp32_band_limited", _lib.PREC_FP32, 0),
+                              ("split_bf16_band_limited", _lib.PREC_BF16X3, 0)):
+            rate, res, mgrid = one_at_a_time(prec, mg)
+            ladder[key] = {"catalogues_per_s": round(rate, 3), "maps_grid": mgrid}
+            results[key] = res
+            log(f"one at a time, {key}: {rate:.2f} catalogues/s (maps grid {mgrid}^3)")
+        a = ladder["plain_fp32_full_grid"]["catalogues_per_s"]
+        b = ladder["plain_fp32_band_limited"]["catalogues_per_s"]
+        c = ladder["split_bf16_band_limited"]["catalogues_per_s"]
+        ladder["split_bf16_serving_slots"] = {"catalogues_per_s": round(value, 3), "slots": args.slots}
+        lfl = value / b
+        ladder["ratios"] = {"band_limiting": round(b / a, 3), "split_bf16": round(c / b, 3),
+                            "slot_overlap": round(value / c, 3),
+                            "like_for_like_vs_plain_fp32_band_limited": round(lfl, 3),
+                            "vs_plain_fp32_reference_structure": round(value / a, 3)}
+        ladder["note"] = ("like for like (the same band-limited maps grid) the split-BF16 serving "
+                          f"path is {lfl:.2f}x the plain FP32 one-at-a-time path" +
+                          ("" if lfl >= 3 else ", below the 3x target"))
+        out["speedup"] = ladder
 
     # ---------------------------------------------------------------- Poisson latency
-    lat = {}
     if args.latency_requests > 0:
         rate = 0.8 * value
         rng = np.random.default_rng(5)
         t = np.cumsum(rng.exponential(1.0 / rate, args.latency_requests))
         reqs = batch(args.latency_requests)
-        for r, a in zip(reqs, t):
-            r.arrival = float(a)
+        for r, a_ in zip(reqs, t):
+            r.arrival = float(a_)
+        t0 = time.perf_counter()
         done = eng.run(reqs)
         lat = latency_summary(done)
         lat["arrival_rate_per_s"] = round(rate, 3)
-        log(f"[bench] Poisson stream at {rate:.1f}/s: {lat}")
-
-    # ---------------------------------------------------------------- mixed-grid stream
-    mixed = None
+        lat["served_per_s"] = round(len(done) / (time.perf_counter() - t0), 3)
+        out["latency"] = lat
+        log(f"Poisson stream at {rate:.1f}/s: {lat}")
     eng.release()
-    if args.mixed_requests > 0:
-        mixed = mixed_stream(ctx, torch, args.mixed_requests, args.mixed_pool, box,
-                             args.mixed_shells, args.mixed_slots)
+    del eng
 
-    # ---------------------------------------------------------------- triangle GEMM alone
-    gemm = gemm_bench(ctx, torch) if args.gemm else None
+    # ---------------------------------------------------------------- cfg4 stream
+    guarded("cfg4", lambda: cfg4_stream(ctx, torch, args, remaining(args) - 170), out, args, 120)
 
-    # ---------------------------------------------------------------- CPU baseline
+    # ---------------------------------------------------------------- precision + CPU baseline
     cpu = None
-    if args.cpu_baseline:
-        cpu = cpu_baseline(pool[0], plan, m, N, box, ctx)
+    if args.cpu_baseline or args.precision:
+        guarded("cpu_cfg2", lambda: cpu_cfg2(pool[0], plan, m, N, results), out, args, 90)
+        c2 = out.pop("cpu_cfg2", {})
+        if "precision" in c2:
+            out["precision"] = c2.pop("precision")
+        if "value" in c2:
+            cpu = c2
+        else:
+            out["cpu_baseline_cfg2"] = c2
+    guarded("cfg1_cpu", lambda: cpu_cfg1(ctx, torch), out, args, 20)
+    if cpu is not None and isinstance(out.get("cfg1_cpu"), dict):
+        cpu["cfg1"] = out.pop("cfg1_cpu")
 
-    out = {
+    # ---------------------------------------------------------------- cfg3, cfg5, GEMM, sort
+    if args.cfg3:
+        guarded("cfg3", lambda: cfg3_leg(ctx, torch), out, args, 45)
+    if args.cfg5:
+        guarded("cfg5", lambda: large_box(ctx, torch, n=args.cfg5), out, args, 60)
+    if args.gemm:
+        guarded("triangle_gemm", lambda: gemm_bench(ctx, torch), out, args, 30)
+    if args.permuted:
+        guarded("permuted_assign", lambda: permuted_assign(ctx, torch, pool[0], plan), out, args, 10)
+
+    res = {
         "metric": METRIC, "value": round(value, 4), "unit": "catalogues/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
@@ -341,114 +407,321 @@ def main():
                    "precision": "split-bf16x3", "parallelism": "single GPU, concurrent slots"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "plain_fp32_catalogues_per_s": None if plain_value is None else round(plain_value, 4),
-        "speedup_vs_plain": None if plain_value is None else round(value / plain_value, 3),
-        "latency": lat,
-        "mixed_stream": mixed,
-        "split_bf16_vs_fp32_err": None if errs is None else
-        {"max": errs[0], "median": errs[1], "p99": errs[2]},
-        "stage_ms_rsd_field": stages,
-        "triangle_gemm": gemm,
     }
-    print(json.dumps(out), flush=True)
+    res.update(out)
+    res["run_seconds"] = round(time.perf_counter() - T_START, 1)
+    res["child_processes_at_exit"] = children_at_exit()
+    line = json.dumps(res, separators=(",", ":"))
+    log(f"JSON line: {len(line)} characters")
+    print(line, flush=True)
     return 0
 
 
-def mixed_stream(ctx, torch, nreq, npool, box, shells=16, slots=4):
-    """Serving a mixed stream: `nreq` requests cycling `npool` resident Zel'dovich catalogues
-    whose particle counts are log-uniform in [1e6, 5e7] (lattice n = round(count^(1/3))) with
-    amplitudes uniform in [1e4, 4e4]; each request draws its grid from {128, 192, 256, 384}^3 and
-    its assignment from CIC/TSC/PCS (interlaced) and asks for redshift-space P_l (l = 0, 2, 4)
-    plus the bispectrum with kbin = kcenter = 2 over `shells` shells (16: up to 32 k_F; 0: the
-    reference's default count for the grid, N / (3 kbin)).  Reported: capacity (all requests
-    queued at t = 0), p50/p99 latency of a Poisson stream at 80% of it, and the single-slot FP32
-    path (full-grid shell maps, FP32 triangle sums) over the same requests."""
+def children_at_exit():
+    """Child processes still alive (pid:name), after waiting up to 5 s for them to end."""
+    try:
+        import psutil
+    except ImportError:
+        return None
+    kids = psutil.Process().children(recursive=True)
+    _, alive = psutil.wait_procs(kids, timeout=5)
+    return [f"{k.pid}:{k.name()}" for k in alive]
+
+
+def cfg4_stream(ctx, torch, args, budget):
+    """Serving configuration: a Poisson stream over --cfg4-pool resident Zel'dovich catalogues
+    (particle counts log-uniform in [1e6, 5e7], lattice n = round(count^(1/3)), amplitudes
+    uniform in [1e4, 4e4]); each request draws its grid from {128, 192, 256, 384}^3 and its
+    assignment from CIC/TSC/PCS (interlaced) and asks for redshift-space P_l (l = 0, 2, 4) and the
+    bispectrum with kbin = kcenter = 2 over the reference's default shell count for the grid,
+    N/(3 kbin) = 21, 32, 42, 64 (pbi4/pbi4.py:99).  A capacity pass (the first 200 requests all
+    queued at t = 0) sets the Poisson rate at 80% of it; the stream's served rate and p50/p99
+    latency are reported, and the plain FP32 path (one slot, one request at a time) over the first
+    --cfg4-fp32 requests.  `budget` seconds bound the stream: fewer requests are sent if the full
+    count would not fit."""
     from pbi4_amd import _lib
     from pbi4_amd.catalogue import zeldovich
     from pbi4_amd.pipeline import Field, Measurement
     from pbi4_amd.serving import Request, ServingEngine, latency_summary
+    f = Field(True, (0.0, 0.0, 1.0))
     rng = np.random.default_rng(2024)
     pool = []
-    for i in range(npool):
+    for i in range(args.cfg4_pool):
         n = int(round(math.exp(rng.uniform(math.log(1e6), math.log(5e7))) ** (1.0 / 3.0)))
         amp = float(rng.uniform(1e4, 4e4))
-        pool.append(zeldovich(n, box=box, amp=amp, seed=7000 + i, device="cuda"))
+        pool.append(zeldovich(n, box=BOX, amp=amp, seed=7000 + i, device="cuda"))
     torch.cuda.synchronize()
-    draws = [(i % npool, int(rng.choice((128, 192, 256, 384))), int(rng.choice((2, 3, 4))))
+    nreq = args.cfg4_requests
+    draws = [(i % len(pool), int(rng.choice((128, 192, 256, 384))), int(rng.choice((2, 3, 4))))
              for i in range(nreq)]
 
-    def meas(grid, order, prec, full):
-        return Measurement(grid=grid, order=order, interlacing=True, box=box, kbin=2.0,
-                           kcenter=2.0, nbins_ps=0, nbins_bisp=shells, iopen=True,
-                           fields=(Field(True, (0.0, 0.0, 1.0)),), precision=prec,
-                           maps_grid=grid if full else 0)
+    def meas(grid, order, prec):
+        return Measurement(grid=grid, order=order, interlacing=True, box=BOX, kbin=2.0,
+                           kcenter=2.0, nbins_ps=0, nbins_bisp=0, iopen=True, fields=(f,),
+                           precision=prec, maps_grid=0)
 
-    def requests(prec, full):
-        out = []
-        for rid, (c, grid, order) in enumerate(draws):
-            p, v = pool[c]
-            out.append(Request(rid, meas(grid, order, prec, full), p, v, arrival=0.0))
-        return out
+    def requests(prec, ids):
+        return [Request(k, meas(draws[k][1], draws[k][2], prec), *pool[draws[k][0]], arrival=0.0)
+                for k in ids]
 
-    def warm(eng, prec, full):
-        # plans (rocFFT plans, exact triangle counts) and one pass per shape, untimed
-        seen = {}
-        for c, grid, order in draws:
-            seen.setdefault((grid, order), c)
-        eng.run([Request(-1, meas(g, o, prec, full), *pool[c], arrival=0.0)
-                 for (g, o), c in seen.items()])
-        torch.cuda.synchronize()
-
-    eng = ServingEngine(ctx, slots=slots)
-    warm(eng, _lib.PREC_BF16X3, False)
-    log("[bench] mixed stream: plans and one pass per shape done")
-    t0 = time.perf_counter()
-    eng.run(requests(_lib.PREC_BF16X3, False))
+    t_start = time.perf_counter()
+    eng = ServingEngine(ctx, slots=args.slots)
+    first = {}
+    for k, (c, g, o) in enumerate(draws):
+        first.setdefault((g, o), k)
+    eng.run(requests(_lib.PREC_BF16X3, sorted(first.values())))       # plans, one pass per shape
     torch.cuda.synchronize()
-    cap = nreq / (time.perf_counter() - t0)
-    log(f"[bench] mixed stream: capacity {cap:.2f}/s")
+    ncap = min(200, nreq)
+    t0 = time.perf_counter()
+    eng.run(requests(_lib.PREC_BF16X3, range(ncap)))
+    torch.cuda.synchronize()
+    cap = ncap / (time.perf_counter() - t0)
     rate = 0.8 * cap
-    reqs = requests(_lib.PREC_BF16X3, False)
-    t = np.cumsum(np.random.default_rng(6).exponential(1.0 / rate, nreq))
+    log(f"cfg4: capacity {cap:.2f}/s over {ncap} queued requests")
+    # the plain path: FP32 sums, one slot, the host waits for each result (the serving slots'
+    # workspaces are released first: a 384^3 request with 64 shells needs ~100 GB)
+    eng.release()
+    torch.cuda.empty_cache()
+    one = ServingEngine(ctx, slots=1)
+    nfp = min(args.cfg4_fp32, nreq)
+    one.run(requests(_lib.PREC_FP32, sorted(first.values())))         # its plans
+    t0 = time.perf_counter()
+    for k in range(nfp):
+        one.run(requests(_lib.PREC_FP32, [k]))
+    plain = nfp / (time.perf_counter() - t0)
+    one.release()
+    del one
+    torch.cuda.empty_cache()
+    log(f"cfg4: plain FP32 one at a time {plain:.2f}/s over {nfp} requests")
+    left = budget - (time.perf_counter() - t_start)
+    nstream = int(min(nreq, max(ncap, rate * left * 0.85)))
+    reqs = requests(_lib.PREC_BF16X3, range(nstream))
+    t = np.cumsum(np.random.default_rng(6).exponential(1.0 / rate, nstream))
     for r, a in zip(reqs, t):
         r.arrival = float(a)
     t0 = time.perf_counter()
-    done = eng.run(reqs)
+    mark = [0]
+
+    def progress(k):
+        if k >= mark[0] + 200:
+            mark[0] = k - k % 200
+            log(f"cfg4: {k} of {nstream} requests served")
+
+    done = eng.run(reqs, progress=progress)
     torch.cuda.synchronize()
-    poisson_rate = nreq / (time.perf_counter() - t0)
+    served = nstream / (time.perf_counter() - t0)
     lat = latency_summary(done)
     eng.release()
     del eng
-    log(f"[bench] mixed stream: Poisson at {rate:.2f}/s served {poisson_rate:.2f}/s")
-    plain = ServingEngine(ctx, slots=1)
-    warm(plain, _lib.PREC_FP32, True)
-    log("[bench] mixed stream: FP32 path warmed")
-    t0 = time.perf_counter()
-    plain.run(requests(_lib.PREC_FP32, True))
-    torch.cuda.synchronize()
-    plain_rate = nreq / (time.perf_counter() - t0)
-    plain.release()
-    del plain
-    counts = [int(pool[c][0].shape[1]) for c, _, _ in draws]
-    res = {"requests": nreq, "distinct_catalogues": npool,
-           "shells": {g: meas(g, 4, _lib.PREC_BF16X3, False).nb_bisp() for g in (128, 192, 256, 384)},
-           "triangles": {g: meas(g, 4, _lib.PREC_BF16X3, False).itot() for g in (128, 192, 256, 384)},
+    counts = [int(pool[c][0].shape[1]) for c, _, _ in draws[:nstream]]
+    res = {"requests": nstream, "requests_asked": nreq, "distinct_catalogues": len(pool),
+           "shells": {g: meas(g, 4, 1).nb_bisp() for g in (128, 192, 256, 384)},
+           "triangles": {g: meas(g, 4, 1).itot() for g in (128, 192, 256, 384)},
            "npart_min": min(counts), "npart_max": max(counts),
-           "grids": {g: sum(1 for d in draws if d[1] == g) for g in (128, 192, 256, 384)},
-           "orders": {o: sum(1 for d in draws if d[2] == o) for o in (2, 3, 4)},
+           "grids": {g: sum(1 for d in draws[:nstream] if d[1] == g) for g in (128, 192, 256, 384)},
+           "orders": {o: sum(1 for d in draws[:nstream] if d[2] == o) for o in (2, 3, 4)},
            "capacity_catalogues_per_s": round(cap, 3),
            "poisson_arrival_rate_per_s": round(rate, 3),
-           "poisson_served_per_s": round(poisson_rate, 3),
-           "latency": lat,
-           "plain_fp32_single_slot_catalogues_per_s": round(plain_rate, 3),
-           "speedup_vs_plain": round(cap / plain_rate, 3)}
-    log(f"[bench] mixed stream: {res}")
+           "poisson_served_per_s": round(served, 3), "latency": lat,
+           "plain_fp32_one_at_a_time_per_s": round(plain, 3), "plain_fp32_requests": nfp,
+           "speedup_vs_plain": round(served / plain, 3) if plain else None}
+    if nstream < nreq:
+        res["note"] = f"{nstream} of {nreq} requests: the full stream would not fit the deadline"
     del pool
     torch.cuda.empty_cache()
+    log(f"cfg4: {res}")
     return res
 
 
-def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
+def cpu_cfg2(cat, plan, m, N, results):
+    """The FP64 oracle (the reference algorithm restated, oracle/) timed in full on the host
+    cores for one catalogue of the headline workload: both fields (real space, redshift space
+    along z), assignment, FFT, fcomb, the 22-FFT multipoles, P_l, all shell maps and all 564
+    triangle sums per field; nothing is extrapolated.  Its B is the FP64 reference of the
+    per-bin errors of the GPU results of the same catalogue (the speedup ladder's split-BF16 and
+    FP32 band-limited results and the FP32 full-grid result)."""
+    from oracle import pbi4_oracle as O
+    cores = O.nthreads()
+    p, v = cat
+    npart = p.shape[1]
+    nb = m.nb_bisp()
+    itot = O.itot_count(nb, m.kbin, m.kcenter, m.iopen)
+    counts = plan.counts.cpu().numpy()
+    T = {}
+    ref = {}
+    t_all = time.perf_counter()
+    for f in m.fields:
+        pos = p.cpu().numpy().astype(np.float64)
+        if f.rsd:
+            pos[2] = pos[2] + v[2].cpu().numpy().astype(np.float64) * np.float64(
+                np.float32(plan.rsd_coef))                    # pbi4.py:154
+        pos = pos.astype(np.float32)
+        pos = ((pos + np.float32(BOX)) % np.float32(BOX)) / np.float32(BOX)
+        pos = np.minimum(pos, np.float32(1.0)).astype(np.float64)
+        tag = "z" if f.rsd else "real"
+        t = time.perf_counter()
+        g = O.assign_double_grid(N, 4, npart, pos)
+        T[f"assign_{tag}"] = time.perf_counter() - t
+        del pos
+        t = time.perf_counter()
+        dcl = O.compute_density(npart, g, N, 4, True, BOX)
+        T[f"density_{tag}"] = time.perf_counter() - t
+        del g
+        d2 = d4 = None
+        if f.rsd:
+            t = time.perf_counter()
+            d2, d4 = O.compute_density_multipoles(dcl, N, BOX, np.array(f.los))
+            T["multipoles_z"] = time.perf_counter() - t
+        t = time.perf_counter()
+        if f.rsd:
+            O.measure_pk_multipoles(N, BOX, m.kbin, m.kcenter, m.nb_ps(), dcl, d2, d4)
+        else:
+            O.measure_pk(N, BOX, m.kbin, m.kcenter, m.nb_ps(), dcl, dcl)
+        T[f"pk_{tag}"] = time.perf_counter() - t
+        t = time.perf_counter()
+        ref[f] = O.measure_bk(nb, N, m.kbin, m.kcenter, m.iopen, m.kf(), itot, npart, counts,
+                              dcl, d2, d4)
+        T[f"bk_{tag}"] = time.perf_counter() - t
+        del dcl, d2, d4
+    total = time.perf_counter() - t_all
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    log(f"CPU FP64 oracle, one full catalogue: {total:.1f} s on {cores} cores ({model})")
+    out = {"value": round(1.0 / total, 6), "unit": "catalogues/s", "cores": cores, "kind": "port",
+           "sample": "one full catalogue of the headline workload: both fields, assignment, FFTs, "
+                     "fcomb, the 22-FFT multipoles, P_l, every shell map and all 564 triangle sums "
+                     "per field, timed without extrapolation",
+           "seconds_per_catalogue": round(total, 2), "cpu_model": model,
+           "stage_seconds": {k: round(x, 3) for k, x in T.items()}}
+    # per-bin B errors of the GPU results of this catalogue (pool[0]) against FP64
+    kf = m.kf()
+    prec = {}
+    per_bin = {}
+    want = (("split_bf16_band_limited", "split_bf16"), ("plain_fp32_band_limited", "fp32"),
+            ("plain_fp32_full_grid", "fp32_full_grid"))
+    for key, name in want:
+        if key not in results or 0 not in results[key]:
+            continue
+        res = results[key][0]
+        errs = {}
+        for f in m.fields:
+            tag = "z" if f.rsd else "real"
+            for row, e in b_errors(ref[f], res[f][1].T, kf).items():
+                errs[f"{row}_{tag}"] = e
+        prec[f"{name}_vs_fp64"] = stats(list(errs.values()))
+        if name == "split_bf16":
+            per_bin[f"{name}_vs_fp64"] = {k: tenths_of_decades(e) for k, e in errs.items()}
+    if "split_bf16_band_limited" in results and "plain_fp32_band_limited" in results:
+        errs = {}
+        for f in m.fields:
+            tag = "z" if f.rsd else "real"
+            a = results["plain_fp32_band_limited"][0][f][1].T
+            b = results["split_bf16_band_limited"][0][f][1].T
+            for row, e in b_errors(a, b, kf).items():
+                errs[f"{row}_{tag}"] = e
+        prec["split_bf16_vs_fp32"] = stats(list(errs.values()))
+        per_bin["split_bf16_vs_fp32"] = {k: tenths_of_decades(e) for k, e in errs.items()}
+    out["precision"] = {
+        "scale": "|B - B_ref| / max(|B_l|, |B0|, sigma_B), sigma_B = sqrt(P1 P2 P3 / (N_tr kf^3)) "
+                 "the bin's Gaussian standard deviation",
+        "summary": prec,
+        "per_bin_encoding": "per row a string of two-digit fields, one per triangle in the "
+                            "reference's loop order: v = round(-10 log10(error)), error = "
+                            "10^(-v/10), 99 = below 1.3e-10",
+        "per_bin": per_bin}
+    log(f"precision vs FP64: {prec}")
+    return out
+
+
+def cpu_cfg1(ctx, torch):
+    """The small CPU-runnable workload timed in full with the FP64 oracle and on the GPU: 128^3
+    Zel'dovich particles, grid 64^3 TSC interlaced, real space, P with kbin = kcenter = 1 (32
+    bins), B with kbin = kcenter = 2 over 8 shells (98 triangles)."""
+    from oracle import pbi4_oracle as O
+    from pbi4_amd.catalogue import zeldovich
+    N, n = 64, 128
+    npart = n ** 3
+    p, _ = zeldovich(n, box=BOX, seed=101, device="cuda", velocities=False)
+    itot = O.itot_count(8, 2.0, 2.0, True)
+    ntr = ctx.count_triangles(8, N, 2.0, 2.0, True, False, itot)
+    counts = torch.from_numpy((ntr.astype(np.float64) * N ** 3).astype(np.float32)).cuda()
+    kf = np.float32(2 * np.pi / BOX)
+    desc = ctx.bk_desc(8, N, 2.0, 2.0, True, kf, itot, npart, False, (0.0, 0.0, 1.0), 1, 0)
+
+    def gpu_one():
+        g = ctx.assign(p, N, 3, True, box=BOX, normalised=False)
+        d = ctx.density(g, N, 3, True, BOX, npart)
+        po = ctx.pk(d, N, BOX, 1.0, 1.0, 32, False).cpu()
+        bo = ctx.bk(d, counts, desc).cpu()
+        return po, bo
+
+    gpu_one()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        gpu_one()
+    gpu_s = (time.perf_counter() - t) / reps
+    pos = np.minimum(((p.cpu().numpy() + np.float32(BOX)) % np.float32(BOX)) / np.float32(BOX),
+                     np.float32(1.0)).astype(np.float64)
+    t = time.perf_counter()
+    dcl = O.compute_density(npart, O.assign_double_grid(N, 3, npart, pos), N, 3, True, BOX)
+    O.measure_pk(N, BOX, 1.0, 1.0, 32, dcl, dcl)
+    O.measure_bk(8, N, 2.0, 2.0, True, kf, itot, npart, counts.cpu().numpy(), dcl)
+    cpu_s = time.perf_counter() - t
+    return {"workload": "128^3 Zel'dovich particles, grid 64^3 TSC interlaced, real space, P "
+                        "(32 bins of k_F), B (8 shells of 2 k_F, 98 triangles)",
+            "cpu_fp64_seconds_per_catalogue": round(cpu_s, 3), "cores": O.nthreads(),
+            "gpu_seconds_per_catalogue": round(gpu_s, 5),
+            "gpu_catalogues_per_s": round(1.0 / gpu_s, 2)}
+
+
+def cfg3_leg(ctx, torch):
+    """Precision form: 512^3 Zel'dovich particles, grid 512^3 PCS interlaced, real space, P with
+    kbin = kcenter = 1 and the bispectrum over 48 shells of dk = k_F (11 228 triangles) on the
+    band-limited 150^3 maps grid: catalogues per second one at a time with split-BF16 and with
+    FP32 triangle sums, and the per-bin difference of the two (the FP64 comparison of sampled
+    triangles is in the GPU tests' cfg3 report)."""
+    from pbi4_amd import _lib
+    from pbi4_amd.catalogue import zeldovich
+    from pbi4_amd.pipeline import CataloguePlan, Field, Measurement
+    n = N = 512
+    p, _ = zeldovich(n, box=BOX, seed=3000, device="cuda", velocities=False)
+    torch.cuda.synchronize()
+    out, res = {}, {}
+    for prec, name in ((_lib.PREC_BF16X3, "split_bf16"), (_lib.PREC_FP32, "fp32")):
+        m = Measurement(grid=N, order=4, interlacing=True, box=BOX, kbin=1.0, kcenter=1.0,
+                        nbins_ps=0, nbins_bisp=48, iopen=True, fields=(Field(False),),
+                        precision=prec, maps_grid=0)
+        plan = CataloguePlan(ctx, m)
+        r = plan.run(p, None, n ** 3)
+        torch.cuda.synchronize()
+        reps = 3
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = plan.run(p, None, n ** 3)
+            torch.cuda.synchronize()
+        out[f"{name}_catalogues_per_s"] = round(reps / (time.perf_counter() - t), 3)
+        res[name] = r[Field(False)][1].cpu().numpy().T
+        out["maps_grid"] = plan.maps_grid
+        out["triangles"] = plan.itot
+    out["split_bf16_speedup_vs_fp32"] = round(out["split_bf16_catalogues_per_s"] /
+                                              out["fp32_catalogues_per_s"], 3)
+    e = b_errors(res["fp32"], res["split_bf16"], np.float32(2 * np.pi / BOX))["B0"]
+    out["split_bf16_vs_fp32"] = stats(e)
+    del p
+    torch.cuda.empty_cache()
+    log(f"cfg3: {out}")
+    return out
+
+
+def large_box(ctx, torch, n=1024, reps=1):
     """The large single-box configuration: n^3 Zel'dovich particles (float32 positions and
     velocities, 26 GB at n = 1024), grid n^3, PCS with interlacing, redshift space along x, y and
     z, P_l (l = 0, 2, 4, kbin = kcenter = 2) and the split-BF16x3 bispectrum over 32 shells of
@@ -458,22 +731,18 @@ def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
     from pbi4_amd.catalogue import rsd_factor, zeldovich
     from pbi4_amd.pipeline import CataloguePlan, Field, Measurement
     N = n
-    log(f"[large] generating {n}^3 Zel'dovich particles on the GPU")
-    p, v = zeldovich(n, box=box, seed=4242, device="cuda")
+    log(f"large: generating {n}^3 Zel'dovich particles on the GPU")
+    p, v = zeldovich(n, box=BOX, seed=4242, device="cuda")
     torch.cuda.synchronize()
     npart = n ** 3
     los3 = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))
-    m = Measurement(grid=N, order=4, interlacing=True, box=box, kbin=2.0, kcenter=2.0,
+    m = Measurement(grid=N, order=4, interlacing=True, box=BOX, kbin=2.0, kcenter=2.0,
                     nbins_ps=0, nbins_bisp=32, iopen=True, fields=tuple(Field(True, l) for l in los3),
                     precision=_lib.PREC_BF16X3, maps_grid=0)
-    t0 = time.perf_counter()
     plan = CataloguePlan(ctx, m)
     torch.cuda.synchronize()
-    log(f"[large] plan (bin tables, rocFFT plans, exact triangle counts) in "
-        f"{time.perf_counter() - t0:.2f} s: itot={plan.itot}, maps grid {plan.maps_grid}^3")
     s = torch.cuda.current_stream()
     coef = float(rsd_factor())
-
     acc = {"dep": [0.0, 0], "gemm": [0.0, 0]}
 
     def field(l, timed):
@@ -482,7 +751,7 @@ def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
         if timed:
             ctx.timing_enable("pbi4_deposit")
         ev[0].record(s)
-        g = ctx.assign(p, N, 4, True, vel=v, box=box, rsd=rsd, normalised=False)
+        g = ctx.assign(p, N, 4, True, vel=v, box=BOX, rsd=rsd, normalised=False)
         ev[1].record(s)
         if timed:
             torch.cuda.synchronize()
@@ -490,10 +759,10 @@ def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
             acc["dep"][0] += ms
             acc["dep"][1] += k
             ctx.timing_enable(None)
-        d = ctx.density(g, N, 4, True, box, npart)
+        d = ctx.density(g, N, 4, True, BOX, npart)
         del g
         ev[2].record(s)
-        po = ctx.pk(d, N, box, 2.0, 2.0, m.nb_ps(), True, l)
+        po = ctx.pk(d, N, BOX, 2.0, 2.0, m.nb_ps(), True, l)
         ev[3].record(s)
         desc = plan.desc[Field(True, l)]
         desc.npart = npart
@@ -512,7 +781,7 @@ def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
 
     field(los3[2], False)                           # warm-up (plans, workspaces)
     t0 = time.perf_counter()
-    for rep in range(reps):
+    for _ in range(reps):
         for l in los3:
             field(l, False)
     torch.cuda.synchronize()
@@ -528,44 +797,21 @@ def large_box(ctx, torch, n=1024, box=1000.0, reps=2):
     dep_gbs = (12.0 * npart + 8.0 * V) / (dep_ms * 1e-3) / 1e9
     nb = m.nb_bisp()
     gemm_tf = 3 * 2.0 * (nb * (nb + 1) // 2) * 3 * nb * plan.maps_grid ** 3 / (gemm_ms * 1e-3) / 1e12
-    out = {"metric": "large box: seconds per catalogue (3 lines of sight, P_l + B)",
-           "value": round(per_cat, 4), "unit": "s", "higher_is_better": False,
-           "config": {"workload": f"{n}^3 Zel'dovich particles, grid {N}^3 PCS interlaced, "
-                                  "redshift space along x, y, z, P_l l=0,2,4, B with 32 shells of "
-                                  "2 k_F (split-BF16x3)", "npart": npart, "itot": plan.itot,
-                      "maps_grid": plan.maps_grid},
+    out = {"workload": f"{n}^3 Zel'dovich particles, grid {N}^3 PCS interlaced, redshift space "
+                       "along x, y, z, P_l l=0,2,4, B with 32 shells of 2 k_F (split-BF16x3)",
+           "seconds_per_catalogue": round(per_cat, 4), "npart": npart, "itot": plan.itot,
+           "maps_grid": plan.maps_grid,
            "stage_ms_per_field": {k: round(float(x), 3) for k, x in
                                   zip(("assign", "density", "pk", "bk"), stages)},
            "deposit": {"avg_launch_ms": round(dep_ms, 3), "algorithmic_GBps": round(dep_gbs, 1),
                        "frac_of_hbm": round(dep_gbs / HBM_PEAK_GBS, 4)},
            "triangle_gemm": {"avg_launch_ms": round(gemm_ms, 3), "executed_tflops": round(gemm_tf, 1),
                              "mfma_frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4)},
-           "pk_rows": int(po.shape[0]), "bk_rows": int(bo.shape[0]),
            "finite": bool(torch.isfinite(po).all() and torch.isfinite(bo).all())}
-    log(f"[large] {out}")
+    log(f"large: {out}")
     del p, v
     torch.cuda.empty_cache()
     return out
-
-
-def triangle_list(nb, kbin, kcenter, iopen):
-    """(l, j, i) of the reference's triangle loop (pbi4/pbi4.py:105-117)."""
-    ishift = int(kcenter / kbin) + (1 if iopen else 0)
-    return [(l, j, i) for l in range(1, nb + 1) for j in range(1, l + 1)
-            for i in range(max(1, l - j + 1 - ishift), j + 1)]
-
-
-def gemm_blocks(nb, nell, kbin, kcenter, iopen):
-    """16 x 16 output blocks of the triangle GEMM that the masked kernel computes (per 16-row
-    fragment of pair rows, the column tiles holding a sum the finalize reads) and the number of
-    sums it reads (the useful entries)."""
-    tiles = {}
-    tl = triangle_list(nb, kbin, kcenter, iopen)
-    for l, j, i in tl:
-        p = (j - 1) * j // 2 + (i - 1)
-        for e in range(nell):
-            tiles.setdefault(p // 16, set()).add((e * nb + l - 1) // 16)
-    return sum(len(t) for t in tiles.values()), len(tl) * nell
 
 
 def gemm_bench(ctx, torch):
@@ -575,7 +821,7 @@ def gemm_bench(ctx, torch):
       dense:         every (pair, column) sum, through pbi4_triangle_sums on random
                      unit-variance maps;
       in_bispectrum: the bispectrum's own GEMM, timed inside pbi4_measure_bk on a random 512^3
-                     half spectrum (executed = the 16 x 16 blocks the kernel issues).
+                     half spectrum.
     executed_tflops counts the MFMA work issued (x3 for split-BF16), useful_tflops only the sums
     that are read; mfma_frac = executed / peak."""
     from pbi4_amd import _lib
@@ -590,7 +836,7 @@ def gemm_bench(ctx, torch):
     ntr = ctx.count_triangles(nb, N, 1.0, 1.0, True, False, itot)
     counts = torch.from_numpy((ntr.astype(np.float64) * float(N) ** 3).astype(np.float32)).cuda()
     delta = torch.randn((N, N, N // 2 + 1), generator=g, device="cuda", dtype=torch.complex64)
-    kf = np.float32(2 * np.pi / 1000.0)
+    kf = np.float32(2 * np.pi / BOX)
 
     def timed(fn, reps=5):
         fn()
@@ -599,9 +845,9 @@ def gemm_bench(ctx, torch):
         for _ in range(reps):
             fn()
         torch.cuda.synchronize()
-        ms, n = ctx.timing_read()
+        ms, k = ctx.timing_read()
         ctx.timing_enable(None)
-        return ms / max(n, 1)
+        return ms / max(k, 1)
 
     for nell in (1, 3):
         maps = torch.randn((nb * nell, Ks), generator=g, device="cuda", dtype=torch.float32)
@@ -621,7 +867,6 @@ def gemm_bench(ctx, torch):
             assert ctx.maps_grid(desc) == Np
             ws = ctx._ws(ctx.bk_workspace(desc))
             avg = timed(lambda: ctx.bk(delta, counts, desc, ws=ws))
-            executed = x * 2.0 * npair * nb * nell * K
             tf = executed / (avg * 1e-3) / 1e12
             res[name + "_in_bispectrum"] = {
                 "avg_ms": round(avg, 4), "executed_tflops": round(tf, 1),
@@ -631,105 +876,41 @@ def gemm_bench(ctx, torch):
             del ws
         out[f"nb48_nell{nell}_K150^3"] = res
         del maps
-    log(f"[bench] triangle GEMM alone: {out}")
+    log(f"triangle GEMM alone: {out}")
     return out
 
 
-def cpu_baseline(cat, plan, m, N, box, ctx):
-    """FP64 oracle (the reference algorithm restated) on the host cores, bounded sample: the
-    redshift-space field of one catalogue in full except the triangle sums, which run on every
-    16th triangle; the real-space field and the other triangles are extrapolated from the
-    measured per-stage times.  Also checks the GPU bispectrum on the sampled triangles."""
-    import torch
-    from oracle import pbi4_oracle as O
-    cores = O.nthreads()
+def permuted_assign(ctx, torch, cat, plan):
+    """Assignment (sort + deposit + gather) and deposit-kernel time of the headline catalogue in
+    its lattice order and randomly permuted (the sort's run aggregation and the deposit's lane
+    hand-out are tuned on spatially ordered input; the grids are bitwise the same either way)."""
     p, v = cat
     npart = p.shape[1]
-    pos = p.cpu().numpy().astype(np.float64)
-    vz = v[2].cpu().numpy().astype(np.float64)
-    pos[2] = pos[2] + vz * (plan.rsd_coef * 1.0)          # pbi4.py:154 (float64 then float32)
-    pos = pos.astype(np.float32)
-    pos = ((pos + np.float32(box)) % np.float32(box)) / np.float32(box)
-    pos = np.minimum(pos, np.float32(1.0)).astype(np.float64)
-    los = np.array([0.0, 0.0, 1.0])
-    T = {}
-    t = time.perf_counter()
-    g = O.assign_double_grid(N, 4, npart, pos)
-    T["assign"] = time.perf_counter() - t
-    del pos
-    t = time.perf_counter()
-    dcl = O.compute_density(npart, g, N, 4, True, box)
-    T["density"] = time.perf_counter() - t
-    del g
-    t = time.perf_counter()
-    d2, d4 = O.compute_density_multipoles(dcl, N, box, los)
-    T["multipoles"] = time.perf_counter() - t
-    t = time.perf_counter()
-    O.measure_pk_multipoles(N, box, m.kbin, m.kcenter, m.nb_ps(), dcl, d2, d4)
-    T["pk"] = time.perf_counter() - t
-    nb = m.nb_bisp()
-    tl, idx = O._triangle_map_index(nb, m.kbin, m.kcenter, m.iopen)
-    sample = np.arange(0, len(tl), 16)
-    t = time.perf_counter()
-    maps, nmodes = O.shell_maps(dcl, N, nb, m.kbin, m.kcenter)
-    m2, _ = O.shell_maps(d2, N, nb, m.kbin, m.kcenter)
-    m4, _ = O.shell_maps(d4, N, nb, m.kbin, m.kcenter)
-    T["maps"] = time.perf_counter() - t
-    del d2, d4
-    t = time.perf_counter()
-    s0 = O.triple_sums(maps, maps, idx[sample])
-    s2 = O.triple_sums(maps, m2, idx[sample])
-    s4 = O.triple_sums(maps, m4, idx[sample])
-    T["sums_sample"] = time.perf_counter() - t
-    full_sums = T["sums_sample"] * len(tl) / len(sample)
-    t_rsd = T["assign"] + T["density"] + T["multipoles"] + T["pk"] + T["maps"] + full_sums
-    t_real = T["assign"] + T["density"] + T["pk"] + T["maps"] / 3 + full_sums / 3
-    total = t_rsd + t_real
-    sampled = sum(T.values())
-    log(f"[bench] CPU oracle stages (s, {cores} cores): {({k: round(x, 2) for k, x in T.items()})}"
-        f" -> {total:.1f} s per catalogue (extrapolated)")
-    # FP64 parity of the GPU path at full size on every triangle of the redshift-space field
-    # (the sums beyond the timed sample are not timed)
-    rest = np.setdiff1d(np.arange(len(tl)), sample)
-    order = np.concatenate([sample, rest])
-    s0 = np.concatenate([s0, O.triple_sums(maps, maps, idx[rest])])[np.argsort(order)]
-    s2 = np.concatenate([s2, O.triple_sums(maps, m2, idx[rest])])[np.argsort(order)]
-    s4 = np.concatenate([s4, O.triple_sums(maps, m4, idx[rest])])[np.argsort(order)]
-    del m2, m4
-    counts = plan.counts.cpu().numpy().astype(np.float64)
-    kf3 = O.kf3_bk(m.kf())
-    f = [k for k in plan.desc if k.rsd][0]
-    check = {}
-    for name, prec, mg in (("split_bf16_bandlimited", 1, 0), ("fp32_full_grid", 0, N)):
-        d = plan.desc[f]
-        gdesc = ctx.bk_desc(nb, N, m.kbin, m.kcenter, m.iopen, m.kf(), plan.itot, npart, True,
-                            (0.0, 0.0, 1.0), prec, mg)
-        rsd = (0.0, 0.0, float(np.float32(plan.rsd_coef)))
-        gg = ctx.assign(p, N, 4, True, vel=v, box=box, rsd=rsd, normalised=False)
-        dd = ctx.density(gg, N, 4, True, box, npart)
-        bo = ctx.bk(dd, plan.counts, gdesc).cpu().numpy()
-        del d
-        ref = {6: kf3 * s0 / counts, 9: 5 * kf3 * s2 / counts, 10: 9 * kf3 * s4 / counts}
-        b0 = np.abs(ref[6])
-        gauss = sigma_b(bo.T, m.kf())
-        worst, worst_plain = 0.0, 0.0
-        for r in (6, 9, 10):
-            mine = bo[:, r] - (bo[:, 7] if r == 6 else 0)
-            err = np.abs(mine - ref[r])
-            worst = max(worst, float((err / np.maximum(np.maximum(np.abs(ref[r]), b0), gauss)).max()))
-            worst_plain = max(worst_plain, float((err / np.maximum(np.abs(ref[r]), b0)).max()))
-        check[name] = {"vs_max_B_sigma": worst, "vs_max_B": worst_plain}
-        del gg, dd
+    rsd = tuple(float(np.float32(plan.rsd_coef * l)) for l in (0.0, 0.0, 1.0))
+    gen = torch.Generator(device=p.device)
+    gen.manual_seed(3)
+    perm = torch.randperm(npart, device=p.device, generator=gen)
+    pp, vp = p[:, perm].contiguous(), v[:, perm].contiguous()
+    out, grids = {}, {}
+    for name, (a, b) in (("lattice_order", (p, v)), ("permuted", (pp, vp))):
+        s = torch.cuda.current_stream()
+        ctx.assign(a, 256, 4, True, vel=b, box=BOX, rsd=rsd, normalised=False)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ctx.timing_enable("pbi4_deposit")
+        ev[0].record(s)
+        reps = 5
+        for _ in range(reps):
+            g = ctx.assign(a, 256, 4, True, vel=b, box=BOX, rsd=rsd, normalised=False)
+        ev[1].record(s)
         torch.cuda.synchronize()
-    log(f"[bench] GPU vs FP64 oracle at full size, all {len(tl)} triangles (B0, B2, B4), max "
-        f"|B - B_oracle| / max(|B_l|, |B0|, sigma_B) and without sigma_B: {check}")
-    return {"value": round(1.0 / total, 6), "unit": "catalogues/s", "cores": cores, "kind": "port",
-            "sample": f"redshift-space field of one 256^3 catalogue (assignment, FFTs, fcomb, 22-FFT "
-                      f"multipoles, P_l, 48 shell maps) in full; triangle sums on {len(sample)} of "
-                      f"{len(tl)} triangles x3; real-space field and other triangles extrapolated; "
-                      f"{sampled:.1f} s measured",
-            "seconds_per_catalogue": round(total, 2), "stage_seconds": {k: round(x, 3) for k, x in T.items()},
-            "gpu_vs_fp64_oracle_max_rel_err_all_triangles": check}
+        dep_ms, k = ctx.timing_read()
+        ctx.timing_enable(None)
+        out[name] = {"assign_ms": round(ev[0].elapsed_time(ev[1]) / reps, 4),
+                     "deposit_ms": round(dep_ms / max(k, 1), 4)}
+        grids[name] = g
+    out["grids_bitwise_equal"] = bool(torch.equal(grids["lattice_order"], grids["permuted"]))
+    log(f"permuted assign: {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -194,16 +194,18 @@ class ServingEngine:
             slot.event = torch.cuda.Event()
             slot.event.record(slot.stream)
 
-    def run(self, requests=None, poll_sleep=50e-6):
+    def run(self, requests=None, poll_sleep=50e-6, progress=None):
         """Submit `requests` (their .arrival are offsets from now) and serve until all are
-        finished.  Returns the finished requests in completion order."""
+        finished.  Returns the finished requests in completion order.  progress: optional
+        callable, given the number of this call's requests finished so far whenever some finish."""
         base = self.now()
         n0 = len(self.finished)
         for r in requests or []:
             r.arrival = base + (r.arrival or 0.0)
             self.queue.append(r)
         while self.queue or self.in_flight():
-            self.poll()
+            if self.poll() and progress is not None:
+                progress(len(self.finished) - n0)
             if self.queue or self.in_flight():
                 time.sleep(poll_sleep)
         return self.finished[n0:]
