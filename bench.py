@@ -156,15 +156,34 @@ def triangle_list(nb, kbin, kcenter, iopen):
 
 
 def gemm_blocks(nb, nell, kbin, kcenter, iopen):
-    """16 x 16 output blocks of the triangle GEMM that hold a sum the finalize reads (per 16-row
-    fragment of pair rows, the column tiles with such a sum) and the number of sums it reads."""
-    tiles = {}
+    """16-row x 16-column MFMA tiles the triangle GEMM issues, dense and block-sparse, and the
+    number of sums the finalize reads.  Block-sparse (the library's rule): columns in (l, ell)
+    order; a wave of 16 rf pair rows spans the fragments between the first and last column its
+    triangles read, and all 4 waves of a work-group multiply the widest span of the four (the
+    dense kernel where that would be more than 80 % of the dense tiles)."""
+    npair = nb * (nb + 1) // 2
+    ncol = nb * nell
+    cf = next(c for c in (1, 2, 3, 4, 6, 8, 9, 12) if c >= -(-ncol // 16))
+    rf = 4 if npair >= 900 else 2 if npair >= 300 else 1
+    if cf >= 12 and rf > 2:
+        rf = 2
+    if npair <= 256 and cf >= 3:
+        rf = 4
+    nblocks = -(-npair // (64 * rf))
+    lo, hi = [1 << 30] * (4 * nblocks), [-1] * (4 * nblocks)
     tl = triangle_list(nb, kbin, kcenter, iopen)
     for l, j, i in tl:
-        p = (j - 1) * j // 2 + (i - 1)
-        for e in range(nell):
-            tiles.setdefault(p // 16, set()).add((e * nb + l - 1) // 16)
-    return sum(len(t) for t in tiles.values()), len(tl) * nell
+        w = ((j - 1) * j // 2 + (i - 1)) // (16 * rf)
+        lo[w] = min(lo[w], (l - 1) * nell // 16)
+        hi[w] = max(hi[w], ((l - 1) * nell + nell - 1) // 16)
+    sparse = 0
+    for b in range(nblocks):
+        spans = [hi[w] - lo[w] + 1 for w in range(4 * b, 4 * b + 4) if hi[w] >= 0]
+        sparse += 4 * rf * max(spans, default=0)
+    dense = 4 * nblocks * rf * cf
+    if 5 * sparse > 4 * dense:      # the library keeps the dense kernel above 80 % of its tiles
+        sparse = dense
+    return {"dense": dense, "block_sparse": sparse, "useful": len(tl) * nell}
 
 
 def guarded(name, fn, out, args, need):
@@ -796,7 +815,9 @@ def large_box(ctx, torch, n=1024, reps=1):
     V = N ** 3
     dep_gbs = (12.0 * npart + 8.0 * V) / (dep_ms * 1e-3) / 1e9
     nb = m.nb_bisp()
-    gemm_tf = 3 * 2.0 * (nb * (nb + 1) // 2) * 3 * nb * plan.maps_grid ** 3 / (gemm_ms * 1e-3) / 1e12
+    dsc = plan.desc[Field(True, los3[0])]
+    tiles = gemm_blocks(nb, 3, dsc.kbin, dsc.kcenter, bool(dsc.iopen))
+    gemm_tf = 3 * 2.0 * 256 * tiles["block_sparse"] * plan.maps_grid ** 3 / (gemm_ms * 1e-3) / 1e12
     out = {"workload": f"{n}^3 Zel'dovich particles, grid {N}^3 PCS interlaced, redshift space "
                        "along x, y, z, P_l l=0,2,4, B with 32 shells of 2 k_F (split-BF16x3)",
            "seconds_per_catalogue": round(per_cat, 4), "npart": npart, "itot": plan.itot,
@@ -806,7 +827,8 @@ def large_box(ctx, torch, n=1024, reps=1):
            "deposit": {"avg_launch_ms": round(dep_ms, 3), "algorithmic_GBps": round(dep_gbs, 1),
                        "frac_of_hbm": round(dep_gbs / HBM_PEAK_GBS, 4)},
            "triangle_gemm": {"avg_launch_ms": round(gemm_ms, 3), "executed_tflops": round(gemm_tf, 1),
-                             "mfma_frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4)},
+                             "mfma_frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4),
+                             "tiles_issued_of_dense": round(tiles["block_sparse"] / tiles["dense"], 4)},
            "finite": bool(torch.isfinite(po).all() and torch.isfinite(bo).all())}
     log(f"large: {out}")
     del p, v
@@ -818,10 +840,12 @@ def gemm_bench(ctx, torch):
     """MFMA utilisation of the triangle GEMM at the precision-config shape: 48 shells (kbin =
     kcenter = 1 at grid 512, max |q| = 48) on the band-limited maps grid 150^3, real space (48
     columns) and redshift space (144 columns).
-      dense:         every (pair, column) sum, through pbi4_triangle_sums on random
-                     unit-variance maps;
-      in_bispectrum: the bispectrum's own GEMM, timed inside pbi4_measure_bk on a random 512^3
-                     half spectrum.
+      all_sums:            every (pair, column) sum, through pbi4_triangle_sums on random
+                           unit-variance maps (dense kernel);
+      in_bispectrum:       the bispectrum's own GEMM (block-sparse: the column windows its
+                           triangles read), timed inside pbi4_measure_bk on a random 512^3 half
+                           spectrum;
+      in_bispectrum_dense: the same with PBI4_TRIANGLE_GEMM=dense.
     executed_tflops counts the MFMA work issued (x3 for split-BF16), useful_tflops only the sums
     that are read; mfma_frac = executed / peak."""
     from pbi4_amd import _lib
@@ -852,28 +876,34 @@ def gemm_bench(ctx, torch):
     for nell in (1, 3):
         maps = torch.randn((nb * nell, Ks), generator=g, device="cuda", dtype=torch.float32)
         maps[:, K:] = 0
-        blocks, useful = gemm_blocks(nb, nell, 1.0, 1.0, True)
+        tiles = gemm_blocks(nb, nell, 1.0, 1.0, True)
         res = {}
         for prec, name, peak, x in ((_lib.PREC_BF16X3, "bf16x3", BF16_PEAK_TFLOPS, 3),
                                     (_lib.PREC_FP32, "fp32", 157.3, 1)):
+            def rates(avg, issued_tiles):
+                ex = x * 2.0 * 256 * issued_tiles * K / (avg * 1e-3) / 1e12
+                return {"avg_ms": round(avg, 4), "executed_tflops": round(ex, 1),
+                        "mfma_frac": round(ex / peak, 4),
+                        "useful_tflops": round(x * 2.0 * tiles["useful"] * K / (avg * 1e-3) / 1e12, 1)}
+
             avg = timed(lambda: ctx.triangle_sums(maps, nb, nell, K, prec))
-            executed = x * 2.0 * npair * nb * nell * K
-            tf = executed / (avg * 1e-3) / 1e12
-            res[name + "_dense"] = {"avg_ms": round(avg, 4), "executed_tflops": round(tf, 1),
-                                    "mfma_frac": round(tf / peak, 4),
-                                    "useful_tflops": round(x * 2.0 * useful * K / (avg * 1e-3) / 1e12, 1)}
+            res[name + "_all_sums"] = rates(avg, tiles["dense"])
             desc = ctx.bk_desc(nb, N, 1.0, 1.0, True, kf, itot, 10 ** 8, nell == 3,
                                (0.0, 0.0, 1.0), prec, 0)
             assert ctx.maps_grid(desc) == Np
-            ws = ctx._ws(ctx.bk_workspace(desc))
-            avg = timed(lambda: ctx.bk(delta, counts, desc, ws=ws))
-            tf = executed / (avg * 1e-3) / 1e12
-            res[name + "_in_bispectrum"] = {
-                "avg_ms": round(avg, 4), "executed_tflops": round(tf, 1),
-                "mfma_frac": round(tf / peak, 4),
-                "useful_tflops": round(x * 2.0 * useful * K / (avg * 1e-3) / 1e12, 1),
-                "blocks_needed_of_dense": round(blocks / (-(-npair // 16) * -(-nb * nell // 16)), 4)}
-            del ws
+            for mode, key in (("dense", "_in_bispectrum_dense"), (None, "_in_bispectrum")):
+                if mode:
+                    os.environ["PBI4_TRIANGLE_GEMM"] = mode
+                try:
+                    ws = ctx._ws(ctx.bk_workspace(desc))
+                    avg = timed(lambda: ctx.bk(delta, counts, desc, ws=ws))
+                finally:
+                    os.environ.pop("PBI4_TRIANGLE_GEMM", None)
+                res[name + key] = rates(avg, tiles["dense"] if mode else tiles["block_sparse"])
+                del ws
+            res[name + "_block_sparse_speedup"] = round(
+                res[name + "_in_bispectrum_dense"]["avg_ms"] / res[name + "_in_bispectrum"]["avg_ms"], 3)
+        res["tiles_issued_of_dense"] = round(tiles["block_sparse"] / tiles["dense"], 4)
         out[f"nb48_nell{nell}_K150^3"] = res
         del maps
     log(f"triangle GEMM alone: {out}")

@@ -650,6 +650,7 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a, int nbins_pk 
     L.K = (int64_t)L.Np * L.Np * L.Np;
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, L.nell, L.K, L.Kstride, d.precision);
+    gemm_set_windows(L.g, triangle_list(d.nbins, d.kbin, d.kcenter, d.iopen, 0), false);
     BinPlan b = make_bin_plan(d.grid, nbins_pk, d.nbins, 0, d.los, 1.0f);
     const size_t work = maps_work(ctx, sh, L.Np, d.grid, L.nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
@@ -702,7 +703,7 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
     const double powSN = 1.0 / (double)d->npart / kf3;
     const double gridD = (double)d->grid * d->grid * d->grid;
     const double ratio = (double)d->grid / (double)L.Np;
-    launch_bk_finalize(d->multipoles ? 11 : 9, d->itot, tri, L.C, L.g.npair, L.g.ncol, d->nbins,
+    launch_bk_finalize(d->multipoles ? 11 : 9, d->itot, tri, L.C, L.g,
                        L.scale_exp, L.shell_sum, sh.nmodes_d, counts, kf3, powSN, gridD,
                        ratio * ratio * ratio, (double)d->kcenter, (double)d->kbin, bout, s);
 }
@@ -735,6 +736,7 @@ BkCrossLayout bk_cross_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     L.K = (int64_t)L.Np * L.Np * L.Np;
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, 2, L.K, L.Kstride, d.precision);
+    gemm_set_windows(L.g, triangle_list(d.nbins, d.kbin, d.kcenter, d.iopen, 1), true);
     BinPlan b = make_bin_plan(d.grid, 0, d.nbins, 0, d.los, 1.0f);
     const size_t work = maps_work(ctx, sh, L.Np, d.grid, nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
@@ -1129,7 +1131,7 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
         volatile float k3 = k2 * kfv;
         const double gridD = (double)d->grid * d->grid * d->grid;
         const double ratio = (double)d->grid / (double)L.Np;
-        launch_bk_cross_finalize(d->itot, tri, L.C, L.g.ncol, nb, L.scale_exp, counts, (double)k3,
+        launch_bk_cross_finalize(d->itot, tri, L.C, L.g, L.scale_exp, counts, (double)k3,
                                  gridD, ratio * ratio * ratio, (double)d->kcenter, (double)d->kbin,
                                  bout, s);
     });

@@ -130,20 +130,37 @@ struct GemmPlan {
     int cf, rf;         // column fragments (of 16), row fragments per wave (of 16)
     int kchunk;         // x-values per work-group
     int nkchunks, nkchunks_pad, nrowblocks;
+    // Block-sparse mode, set by gemm_set_windows from a triangle list: logical column c is map
+    // moff + (c % mdiv) * nb + c / mdiv, and wave w (16 rf pair rows) multiplies only the
+    // logical column fragments [wstart[w], wstart[w] + wwidth[w]) that its triangles read.  C is
+    // then npair x ncol_l in logical columns and holds only those entries.
+    bool win = false;
+    int cfw = 0;        // widest window, rounded up to a compiled width
+    int maxw = 0;       // widest window (column fragments): partial rows are 16 maxw long
+    int ucap = 0;       // most column fragments one work-group stages (union of its 4 waves)
+    int moff = 0, mdiv = 1, ncol_l = 0;
+    std::vector<uint8_t> wstart, wwidth;
+    // finalize indexing: auto triangle (l, ell) reads C column (l - 1) * cs_l + ell * cs_e; cross
+    // triangle i reads column coff + i - 1; cld is the C row length
+    int cs_l = 1, cs_e = 0, coff = 0, cld = 0;
 };
 GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision);
+// Switch plan g to block-sparse mode for the triangles in tri ((l, j, i) triples, 1-based, as
+// triangle_list writes them); cross: the cross-bispectrum layout (pair rows (j <= l) of field 1,
+// field-2 map i).  Keeps the dense kernel where the windows do not fit the compiled tiles.
+void gemm_set_windows(GemmPlan &g, const std::vector<int32_t> &tri, bool cross);
 size_t gemm_partials_floats(const GemmPlan &g);
 void launch_gemm(const GemmPlan &g, const float *maps, float *partials, hipStream_t s,
                  KernelTimer *timer);
 void launch_gemm_reduce(const GemmPlan &g, const float *partials, double *C, hipStream_t s);
 
-void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C, int npair,
-                        int ncol, int nb, const float *scale_exp, const double *shell_sum,
+void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C,
+                        const GemmPlan &g, const float *scale_exp, const double *shell_sum,
                         const double *nmodes, const float *counts, double kf3, double powSN,
                         double gridD, double sum_scale, double kcenter, double kbin, double *bout,
                         hipStream_t s);
 
-void launch_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, int ncol, int nb,
+void launch_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, const GemmPlan &g,
                               const float *scale_exp, const float *counts, double kf3,
                               double gridD, double sum_scale, double kcenter, double kbin,
                               double *bout, hipStream_t s);

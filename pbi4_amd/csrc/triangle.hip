@@ -9,6 +9,9 @@
 //   PBI4_PREC_BF16X3: a = a_hi + a_lo, b = b_hi + b_lo in bf16 (round-to-nearest), and
 //                     a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi on v_mfma_f32_16x16x32_bf16 with
 //                     FP32 accumulation (relative representation error ~2^-17 per product).
+#include <cstdlib>
+#include <cstring>
+
 #include "common.h"
 
 namespace pbi4 {
@@ -231,6 +234,301 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm(const float *map
             }
 }
 
+// ------------------------------------------------------------------ block-sparse triangle GEMM
+// Triangle (l, j, i) reads only columns l >= j of pair (i, j), and with the columns in (l, ell)
+// order those form one short run per wave of pair rows: about a third of the dense tile at 48
+// shells x 3 multipoles.  Each work-group (4 waves, 64 RF consecutive pair rows) multiplies W
+// column fragments per wave, W the widest of its waves' windows, each wave from its own first
+// fragment.  Kernel argument, filled on the host from the triangle list:
+constexpr int kMaxWinWaves = 1024;
+struct GemmWindows {
+    uint8_t start[kMaxWinWaves];       // first logical column fragment of each wave's window
+    uint8_t width[kMaxWinWaves / 4];   // W of each work-group (0: no triangle reads its rows)
+};
+
+constexpr int kWinKT = 32;         // x-values per LDS stage (one bf16 MFMA k-step)
+constexpr int kWinMaxMaps = 64;    // A maps staged per work-group: nb <= 64 (2 float4 per thread)
+
+template <int CFW>
+struct WinTile {
+    static constexpr int NVB = CFW / 2 + 2;    // B float4 per thread: staged union <= 2 NVB tiles
+    static constexpr int UCAP = 2 * NVB;
+};
+
+template <int PREC>
+__host__ __device__ constexpr size_t win_stage_bytes(int nb, int ucap) {
+    return sizeof(float) * nb * (kWinKT + kPadF) +
+           (PREC == PBI4_PREC_BF16X3 ? 2 * sizeof(__bf16) * ucap * 16 * (kWinKT + kPadB)
+                                     : sizeof(float) * ucap * 16 * (kWinKT + kPadF));
+}
+
+// One work-group of the block-sparse GEMM with W column fragments per wave (straight-line MFMA
+// code).  Same products and MFMA sequence per C entry as k_triangle_gemm (A = M_i M_j split
+// once per x, B split when staged, k-steps of 32 in x order), so every entry is bitwise the
+// dense kernel's.  The work-group stages the FP32 maps 0 .. jmax-1 its pair rows take (the A
+// sources) and the union of its waves' windows (B).  Partials: [chunk][row][nw], the column
+// offset relative to the wave's window.
+template <int PREC, int W, int NVB, int RF>
+__device__ __forceinline__ void win_body(const float *maps, int64_t K, int64_t Kstride, int nb,
+                                         int moff, int mdiv, int ncol, int npair, int kchunk,
+                                         int nrowblocks, int rowblock, int chunk, int ucap, int nw,
+                                         const GemmWindows &win, float *partials) {
+    constexpr int KT = kWinKT, LDF = KT + kPadF, LDB = KT + kPadB;
+    constexpr int NVA = kWinMaxMaps * (KT / 4) / kGemmThreads;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const size_t abytes = sizeof(float) * nb * LDF;
+    const size_t stage = win_stage_bytes<PREC>(nb, ucap);
+
+    const int lane = threadIdx.x & 63, grp = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wid = rowblock * 4 + wave;
+    int u0 = 255, u1 = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        u0 = min(u0, (int)win.start[rowblock * 4 + v]);
+        u1 = max(u1, (int)win.start[rowblock * 4 + v] + W);
+    }
+    const int nstage = (u1 - u0) * 16;          // staged B columns
+    const int t0 = win.start[wid] - u0;         // the wave's first fragment in the staged tile
+    const int64_t kbeg = (int64_t)chunk * kchunk;
+    const int64_t kend = min(K, kbeg + kchunk);
+    int ilast, jlast;
+    pair_of(min(npair, (rowblock + 1) * 4 * RF * 16) - 1, ilast, jlast);
+    const int amax = jlast + 1;                 // pair rows (i <= j) read maps 0 .. jlast
+
+    int pi[RF], pj[RF];
+#pragma unroll
+    for (int r = 0; r < RF; ++r) {
+        const int prow = wid * RF * 16 + r * 16 + (lane & 15);
+        pi[r] = pj[r] = 0;
+        if (prow < npair) pair_of(prow, pi[r], pj[r]);
+    }
+    f32x4 acc[RF][W];
+#pragma unroll
+    for (int r = 0; r < RF; ++r)
+#pragma unroll
+        for (int c = 0; c < W; ++c) acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    // staging: thread t moves x-quad t % 8 of A map (t / 8) + 32 v and of staged column
+    // (t / 8) + 32 v; the map behind a staged column is fixed for the whole kernel
+    const int xq = threadIdx.x & 7, row0 = threadIdx.x >> 3;
+    const float *asrc[NVA], *bsrc[NVB];
+#pragma unroll
+    for (int v = 0; v < NVA; ++v) {
+        const int m = row0 + 32 * v;
+        asrc[v] = m < amax ? maps + (int64_t)m * Kstride + 4 * xq : nullptr;
+    }
+#pragma unroll
+    for (int v = 0; v < NVB; ++v) {
+        const int u = row0 + 32 * v, c = u0 * 16 + u;
+        bsrc[v] = (u < nstage && c < ncol)
+                      ? maps + (int64_t)(moff + (c % mdiv) * nb + c / mdiv) * Kstride + 4 * xq
+                      : nullptr;
+    }
+    float4 ra[NVA], rb[NVB];
+    auto load = [&](int64_t x0) {
+        const bool in = x0 + 4 * xq < kend;
+#pragma unroll
+        for (int v = 0; v < NVA; ++v) {
+            ra[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (asrc[v] && in) ra[v] = *reinterpret_cast<const float4 *>(asrc[v] + x0);
+        }
+#pragma unroll
+        for (int v = 0; v < NVB; ++v) {
+            rb[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (bsrc[v] && in) rb[v] = *reinterpret_cast<const float4 *>(bsrc[v] + x0);
+        }
+    };
+    auto store = [&](int buf) {
+        unsigned char *base = smem + buf * stage;
+        float *tf = reinterpret_cast<float *>(base);
+#pragma unroll
+        for (int v = 0; v < NVA; ++v)
+            if (row0 + 32 * v < amax) *reinterpret_cast<float4 *>(tf + (row0 + 32 * v) * LDF + 4 * xq) = ra[v];
+#pragma unroll
+        for (int v = 0; v < NVB; ++v) {
+            const int u = row0 + 32 * v;
+            if (u >= nstage) continue;
+            if (PREC == PBI4_PREC_BF16X3) {
+                __bf16 *bhi = reinterpret_cast<__bf16 *>(base + abytes);
+                __bf16 *blo = bhi + ucap * 16 * LDB;
+                uint32_t h[2], l[2];
+                split2(rb[v].x, rb[v].y, h[0], l[0]);
+                split2(rb[v].z, rb[v].w, h[1], l[1]);
+                *reinterpret_cast<uint2 *>(bhi + u * LDB + 4 * xq) = make_uint2(h[0], h[1]);
+                *reinterpret_cast<uint2 *>(blo + u * LDB + 4 * xq) = make_uint2(l[0], l[1]);
+            } else {
+                float *tb = reinterpret_cast<float *>(base + abytes);
+                *reinterpret_cast<float4 *>(tb + u * LDF + 4 * xq) = rb[v];
+            }
+        }
+    };
+    auto compute = [&](int buf) {
+        const unsigned char *base = smem + buf * stage;
+        const float *tf = reinterpret_cast<const float *>(base);
+        if (PREC == PBI4_PREC_BF16X3) {
+            const __bf16 *bhi = reinterpret_cast<const __bf16 *>(base + abytes);
+            const __bf16 *blo = bhi + ucap * 16 * LDB;
+            const int xo = 8 * grp;
+            // the wave's W B fragments in registers, then per row fragment: form A (VALU) and
+            // issue its 3 W MFMAs, so that the next fragment's products fill the MFMA shadow
+            bf16x8 bh[W], bl[W];
+            const __bf16 *bh0 = bhi + (t0 * 16 + (lane & 15)) * LDB + xo;
+            const __bf16 *bl0 = blo + (t0 * 16 + (lane & 15)) * LDB + xo;
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                bh[c] = *reinterpret_cast<const bf16x8 *>(bh0 + c * 16 * LDB);
+                bl[c] = *reinterpret_cast<const bf16x8 *>(bl0 + c * 16 * LDB);
+            }
+#pragma unroll
+            for (int r = 0; r < RF; ++r) {
+                const float4 *ai = reinterpret_cast<const float4 *>(tf + pi[r] * LDF + xo);
+                const float4 *aj = reinterpret_cast<const float4 *>(tf + pj[r] * LDF + xo);
+                const float4 a0 = ai[0], a1 = ai[1], b0 = aj[0], b1 = aj[1];
+                uint32_t h[4], l[4];
+                split2(a0.x * b0.x, a0.y * b0.y, h[0], l[0]);
+                split2(a0.z * b0.z, a0.w * b0.w, h[1], l[1]);
+                split2(a1.x * b1.x, a1.y * b1.y, h[2], l[2]);
+                split2(a1.z * b1.z, a1.w * b1.w, h[3], l[3]);
+                const bf16x8 ahi = __builtin_bit_cast(bf16x8, h);
+                const bf16x8 alo = __builtin_bit_cast(bf16x8, l);
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bh[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bl[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bh[c], acc[r][c], 0, 0, 0);
+                }
+            }
+        } else {
+            const float *tb = reinterpret_cast<const float *>(base + abytes);
+#pragma unroll
+            for (int ks = 0; ks < KT; ks += 16) {
+                const int xo = ks + 4 * grp;
+                float pr[RF][4];
+#pragma unroll
+                for (int r = 0; r < RF; ++r) {
+                    const float4 a = *reinterpret_cast<const float4 *>(tf + pi[r] * LDF + xo);
+                    const float4 b = *reinterpret_cast<const float4 *>(tf + pj[r] * LDF + xo);
+                    pr[r][0] = a.x * b.x;
+                    pr[r][1] = a.y * b.y;
+                    pr[r][2] = a.z * b.z;
+                    pr[r][3] = a.w * b.w;
+                }
+                const float *b0 = tb + (t0 * 16 + (lane & 15)) * LDF + xo;
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    const float4 bv = *reinterpret_cast<const float4 *>(b0 + c * 16 * LDF);
+#pragma unroll
+                    for (int r = 0; r < RF; ++r) {
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[r][0], bv.x, acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[r][1], bv.y, acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[r][2], bv.z, acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[r][3], bv.w, acc[r][c], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    };
+
+    int buf = 0;
+    load(kbeg);
+    store(0);
+    __syncthreads();
+    for (int64_t x0 = kbeg; x0 < kend; x0 += KT) {
+        const bool more = x0 + KT < kend;
+        if (more) load(x0 + KT);
+        compute(buf);
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    const int64_t npair_pad = (int64_t)nrowblocks * 4 * RF * 16;
+    float *out = partials + (int64_t)chunk * npair_pad * nw;
+#pragma unroll
+    for (int r = 0; r < RF; ++r)
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t row = (int64_t)wid * RF * 16 + r * 16 + 4 * grp + i;
+                out[row * nw + c * 16 + (lane & 15)] = acc[r][c][i];
+            }
+}
+
+// Dispatches each work-group on its window width W (1 .. CFW), so that the MFMA loop of every
+// width is straight-line code; the kernel's registers are those of the widest case.
+template <int PREC, int CFW, int RF>
+__global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm_win(
+    const float *maps, int64_t K, int64_t Kstride, int nb, int moff, int mdiv, int ncol, int npair,
+    int kchunk, int nrowblocks, int nchunks, int ucap, int nw, GemmWindows win, float *partials) {
+    constexpr int NVB = WinTile<CFW>::NVB;
+    const int w = blockIdx.x;
+    const int q = w / kXcds;
+    const int rowblock = q % nrowblocks;
+    const int chunk = (w % kXcds) + kXcds * (q / nrowblocks);
+    if (chunk >= nchunks) return;
+    const int W = win.width[rowblock];
+#define PBI4_WIN_CASE(n)                                                                       \
+    case n:                                                                                    \
+        if constexpr (n <= CFW)                                                                \
+            win_body<PREC, n, NVB, RF>(maps, K, Kstride, nb, moff, mdiv, ncol, npair, kchunk,  \
+                                       nrowblocks, rowblock, chunk, ucap, nw, win, partials);  \
+        break;
+    switch (W) {
+        PBI4_WIN_CASE(1)
+        PBI4_WIN_CASE(2)
+        PBI4_WIN_CASE(3)
+        PBI4_WIN_CASE(4)
+        PBI4_WIN_CASE(5)
+        PBI4_WIN_CASE(6)
+        PBI4_WIN_CASE(7)
+        PBI4_WIN_CASE(8)
+        default: break;
+    }
+#undef PBI4_WIN_CASE
+}
+
+// Split-K combine of the block-sparse partials, entries in the waves' windows only: C[p][col]
+// for col = 16 start(wave) + offset, same chunk split and summation order as k_gemm_reduce.
+__global__ __launch_bounds__(256) void k_gemm_reduce_win(const float *partials, int nchunks,
+                                                         int64_t npair_pad, int nw, int wave_rows,
+                                                         int npair, int ncol, GemmWindows win,
+                                                         double *C) {
+    __shared__ double red[8][32];
+    const int lo = threadIdx.x & 31, part = threadIdx.x >> 5;
+    const int64_t t = (int64_t)blockIdx.x * 32 + lo;
+    const int64_t p = t / nw;
+    const int cr = (int)(t % nw);
+    bool ok = p < npair;
+    int col = 0;
+    if (ok) {
+        const int wv = (int)(p / wave_rows);
+        col = 16 * win.start[wv] + cr;
+        ok = cr < 16 * win.width[wv / 4] && col < ncol;
+    }
+    double s = 0.0;
+    if (ok) {
+        const int per = (nchunks + 7) / 8;
+        const int k0 = part * per, k1 = min(nchunks, k0 + per);
+        int k = k0;
+        for (; k + 8 <= k1; k += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partials[((int64_t)(k + u) * npair_pad + p) * nw + cr];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += (double)v[u];
+        }
+        for (; k < k1; ++k) s += (double)partials[((int64_t)k * npair_pad + p) * nw + cr];
+    }
+    red[part][lo] = s;
+    __syncthreads();
+    if (part == 0 && ok) {
+        double r = red[0][lo];
+        for (int q = 1; q < 8; ++q) r += red[q][lo];
+        C[p * ncol + col] = r;
+    }
+}
+
 // Split-K combine in FP64: each block owns 32 consecutive C entries; the chunk range of every
 // entry is split over 8 threads in fixed sub-ranges, then summed in a fixed order in LDS.
 __global__ __launch_bounds__(256) void k_gemm_reduce(const float *partials, int nchunks,
@@ -266,7 +564,8 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const float *partials, int 
 // One thread per triangle m (reference order): bout(:, m) of bispectrumI4.f90:173-182 / :489-500
 // with pow(s) from Parseval and the triple sums unscaled from the band-limited maps grid.
 __global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C,
-                              int ncol, int nb, const float *scale_exp, const double *shell_sum,
+                              int cld, int cs_l, int cs_e, const float *scale_exp,
+                              const double *shell_sum,
                               const double *nmodes, const float *counts, double kf3, double powSN,
                               double gridD, double sum_scale, double kcenter, double kbin,
                               double *bout) {
@@ -282,7 +581,8 @@ __global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const 
         sum_scale * ldexp(1.0, -(int)(scale_exp[i - 1] + scale_exp[j - 1] + scale_exp[l - 1]));
     const double cnt = (double)counts[m];
     double *b = bout + (int64_t)rows * m;
-    const double sum = C[p * ncol + (l - 1)] * unscale;
+    const double *cp = C + p * cld + (int64_t)(l - 1) * cs_l;
+    const double sum = cp[0] * unscale;
     b[0] = kcenter + (double)(l - 1) * kbin;
     b[1] = kcenter + (double)(j - 1) * kbin;
     b[2] = kcenter + (double)(i - 1) * kbin;
@@ -293,8 +593,8 @@ __global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const 
     b[7] = bispSN;
     b[8] = (double)(float)(cnt / gridD);
     if (rows == 11) {
-        const double s2 = C[p * ncol + nb + (l - 1)] * unscale;
-        const double s4 = C[p * ncol + 2 * nb + (l - 1)] * unscale;
+        const double s2 = cp[cs_e] * unscale;
+        const double s4 = cp[2 * cs_e] * unscale;
         b[9] = 5.0 * kf3 * s2 / cnt;
         b[10] = 9.0 * kf3 * s4 / cnt;
     }
@@ -303,8 +603,8 @@ __global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const 
 // Cross bispectrum (bispectrumI4.f90:733-755): triangle m = (l, j, i) takes the pair (j <= l)
 // of field-1 maps (GEMM rows) and field-2 map i (GEMM column nb + i - 1); rows k_l, k_j, k_i,
 // kf3 sum / N_tr, N_tr / N^3.  scale_exp holds the field-1 scales, then the field-2 scales.
-__global__ void k_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, int ncol,
-                                    int nb, const float *scale_exp, const float *counts,
+__global__ void k_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, int cld,
+                                    int coff, int nb, const float *scale_exp, const float *counts,
                                     double kf3, double gridD, double sum_scale, double kcenter,
                                     double kbin, double *bout) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,7 +614,7 @@ __global__ void k_bk_cross_finalize(int64_t itot, const int32_t *tri, const doub
     const double unscale =
         sum_scale * ldexp(1.0, -(int)(scale_exp[j - 1] + scale_exp[l - 1] + scale_exp[nb + i - 1]));
     const double cnt = (double)counts[m];
-    const double sum = C[p * ncol + nb + (i - 1)] * unscale;
+    const double sum = C[p * cld + coff + (i - 1)] * unscale;
     double *b = bout + 5 * m;
     b[0] = kcenter + (double)(l - 1) * kbin;
     b[1] = kcenter + (double)(j - 1) * kbin;
@@ -411,11 +711,149 @@ GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precis
     g.kchunk = (int)kc;
     g.nkchunks = (int)ceil_div(K, kc);
     g.nkchunks_pad = (int)align_up(g.nkchunks, kXcds);
+    g.ncol_l = g.cld = g.ncol;
+    g.cs_l = 1;
+    g.cs_e = nb;
     return g;
 }
 
 size_t gemm_partials_floats(const GemmPlan &g) {
-    return (size_t)g.nkchunks * g.nrowblocks * 64 * g.rf * g.cf * 16;
+    return (size_t)g.nkchunks * g.nrowblocks * 64 * g.rf * (g.win ? g.maxw : g.cf) * 16;
+}
+
+void gemm_set_windows(GemmPlan &g, const std::vector<int32_t> &tri, bool cross) {
+    g.win = false;
+    g.ncol_l = cross ? g.nb : g.nb * g.nell;
+    g.moff = cross ? g.nb : 0;
+    g.mdiv = cross ? 1 : g.nell;
+    // dense indexing (maps in ell-major order; the cross field-2 maps after the field-1 maps)
+    g.cld = g.ncol;
+    g.cs_l = 1;
+    g.cs_e = g.nb;
+    g.coff = cross ? g.nb : 0;
+    // PBI4_TRIANGLE_GEMM=dense keeps the dense kernel, =block_sparse refuses shapes that the
+    // block-sparse tiles do not cover (comparisons and timing of the two)
+    const char *mode = std::getenv("PBI4_TRIANGLE_GEMM");
+    if (mode && std::strcmp(mode, "dense") == 0) return;
+    const bool force = mode && std::strcmp(mode, "block_sparse") == 0;
+    auto dense = [&](const char *why) {
+        PBI4_CHECK(!force, PBI4_EUNSUPPORTED,
+                   std::string("PBI4_TRIANGLE_GEMM=block_sparse: ") + why);
+    };
+    const int nwaves = g.nrowblocks * 4, wave_rows = 16 * g.rf;
+    if (g.nb > kWinMaxMaps) return dense("more than 64 shells");
+    if (nwaves > kMaxWinWaves) return dense("too many pair rows");
+    std::vector<int> lo(nwaves, 1 << 30), hi(nwaves, -1);
+    for (size_t m = 0; m + 2 < tri.size(); m += 3) {
+        const int64_t l = tri[m], j = tri[m + 1], i = tri[m + 2];
+        int64_t p, c0, c1;
+        if (cross) {            // pair (j <= l) of field 1, field-2 column i
+            p = (l - 1) * l / 2 + (j - 1);
+            c0 = c1 = i - 1;
+        } else {                // pair (i <= j), columns (l, ell) for every ell
+            p = (j - 1) * j / 2 + (i - 1);
+            c0 = (l - 1) * g.nell;
+            c1 = c0 + g.nell - 1;
+        }
+        const int w = (int)(p / wave_rows);
+        lo[w] = std::min<int>(lo[w], (int)(c0 / 16));
+        hi[w] = std::max<int>(hi[w], (int)(c1 / 16));
+    }
+    // work-group width W = its widest wave window; every wave multiplies W fragments from its
+    // own first one, moved left where the window would pass the last logical fragment
+    const int cfl = (int)ceil_div(g.ncol_l, 16);
+    std::vector<uint8_t> start(nwaves, 0), width(g.nrowblocks, 0);
+    int maxw = 0, ucap = 0;
+    for (int b = 0; b < g.nrowblocks; ++b) {
+        int W = 0, first = -1;
+        for (int w = 4 * b; w < 4 * b + 4; ++w)
+            if (hi[w] >= 0) W = std::max(W, hi[w] - lo[w] + 1);
+        if (W == 0) continue;
+        for (int w = 4 * b; w < 4 * b + 4; ++w)
+            if (hi[w] >= 0) {
+                start[w] = (uint8_t)std::min(lo[w], cfl - W);
+                if (first < 0) first = start[w];
+            }
+        int u0 = 1 << 30, u1 = -1;
+        for (int w = 4 * b; w < 4 * b + 4; ++w) {
+            if (hi[w] < 0) start[w] = (uint8_t)first;     // rows without triangles: no new tiles
+            u0 = std::min<int>(u0, start[w]);
+            u1 = std::max<int>(u1, start[w] + W);
+        }
+        width[b] = (uint8_t)W;
+        maxw = std::max(maxw, W);
+        ucap = std::max(ucap, u1 - u0);
+    }
+    int cfw = 0;
+    for (int c : {2, 3, 4, 6, 8})
+        if (c >= maxw) {
+            cfw = c;
+            break;
+        }
+    if (maxw == 0) return dense("no triangles");
+    if (cfw == 0) return dense("a window wider than 8 column tiles");
+    const int ucap_max = cfw == 2   ? WinTile<2>::UCAP
+                         : cfw == 3 ? WinTile<3>::UCAP
+                         : cfw == 4 ? WinTile<4>::UCAP
+                         : cfw == 6 ? WinTile<6>::UCAP
+                                    : WinTile<8>::UCAP;
+    if (ucap > ucap_max) return dense("a work-group's windows span too many column tiles");
+    // a work-group with every column tile gains nothing over the dense kernel's wider x-stages
+    // (64 x-values for up to 6 tiles) and single staging of each map
+    int64_t tiles = 0;
+    for (int b = 0; b < g.nrowblocks; ++b) tiles += width[b];
+    if (!force && 5 * tiles > 4 * (int64_t)g.nrowblocks * g.cf) return;
+    g.win = true;
+    g.cfw = cfw;
+    g.maxw = maxw;
+    g.ucap = ucap;
+    g.wstart = std::move(start);
+    g.wwidth = std::move(width);
+    g.cld = g.ncol_l;
+    g.cs_l = cross ? 1 : g.nell;
+    g.cs_e = 1;
+    g.coff = 0;
+}
+
+static GemmWindows windows_arg(const GemmPlan &g) {
+    GemmWindows w{};
+    std::copy(g.wstart.begin(), g.wstart.end(), w.start);
+    std::copy(g.wwidth.begin(), g.wwidth.end(), w.width);
+    return w;
+}
+
+template <int PREC, int CFW, int RF>
+static void launch_win(const GemmPlan &g, const float *maps, float *partials, hipStream_t s) {
+    const size_t lds = 2 * win_stage_bytes<PREC>(g.nb, g.ucap);
+    PBI4_CHECK(lds <= 160 * 1024, PBI4_EUNSUPPORTED, "triangle GEMM tile does not fit in LDS");
+    auto kern = k_triangle_gemm_win<PREC, CFW, RF>;
+    if (lds > 64 * 1024)
+        PBI4_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+    const unsigned nwg = (unsigned)((int64_t)g.nrowblocks * g.nkchunks_pad);
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(kGemmThreads), lds, s, maps, g.K, g.Kstride, g.nb,
+                       g.moff, g.mdiv, g.ncol_l, g.npair, g.kchunk, g.nrowblocks, g.nkchunks,
+                       g.ucap, g.maxw * 16, windows_arg(g), partials);
+}
+
+template <int PREC, int CFW>
+static void launch_win_rf(const GemmPlan &g, const float *maps, float *partials, hipStream_t s) {
+    switch (g.rf) {
+        case 1: launch_win<PREC, CFW, 1>(g, maps, partials, s); break;
+        case 2: launch_win<PREC, CFW, 2>(g, maps, partials, s); break;
+        default: launch_win<PREC, CFW, 4>(g, maps, partials, s); break;
+    }
+}
+
+template <int PREC>
+static void launch_win_cfw(const GemmPlan &g, const float *maps, float *partials, hipStream_t s) {
+    switch (g.cfw) {
+        case 2: launch_win_rf<PREC, 2>(g, maps, partials, s); break;
+        case 3: launch_win_rf<PREC, 3>(g, maps, partials, s); break;
+        case 4: launch_win_rf<PREC, 4>(g, maps, partials, s); break;
+        case 6: launch_win_rf<PREC, 6>(g, maps, partials, s); break;
+        default: launch_win_rf<PREC, 8>(g, maps, partials, s); break;
+    }
 }
 
 template <int PREC, int CF, int RF>
@@ -458,41 +896,54 @@ static void launch_cf(const GemmPlan &g, const float *maps, float *partials, hip
 void launch_gemm(const GemmPlan &g, const float *maps, float *partials, hipStream_t s,
                  KernelTimer *timer) {
     TimedScope ts(timer, "pbi4_triangle_gemm", s);
-    if (g.precision == PBI4_PREC_BF16X3)
+    if (g.win) {
+        if (g.precision == PBI4_PREC_BF16X3)
+            launch_win_cfw<PBI4_PREC_BF16X3>(g, maps, partials, s);
+        else
+            launch_win_cfw<PBI4_PREC_FP32>(g, maps, partials, s);
+    } else if (g.precision == PBI4_PREC_BF16X3) {
         launch_cf<PBI4_PREC_BF16X3>(g, maps, partials, s);
-    else
+    } else {
         launch_cf<PBI4_PREC_FP32>(g, maps, partials, s);
+    }
     PBI4_HIP(hipGetLastError());
 }
 
 void launch_gemm_reduce(const GemmPlan &g, const float *partials, double *C, hipStream_t s) {
-    const int cf = g.cf;
-    const int64_t n = (int64_t)g.npair * g.ncol;
-    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)ceil_div(n, 32)), dim3(256), 0, s, partials,
-                       g.nkchunks, (int64_t)g.nrowblocks * 64 * g.rf, cf * 16, g.npair, g.ncol, C);
+    const int64_t npair_pad = (int64_t)g.nrowblocks * 64 * g.rf;
+    if (g.win) {
+        const int nw = g.maxw * 16;
+        const int64_t n = (int64_t)g.npair * nw;
+        hipLaunchKernelGGL(k_gemm_reduce_win, dim3((unsigned)ceil_div(n, 32)), dim3(256), 0, s,
+                           partials, g.nkchunks, npair_pad, nw, 16 * g.rf, g.npair, g.ncol_l,
+                           windows_arg(g), C);
+    } else {
+        const int64_t n = (int64_t)g.npair * g.ncol;
+        hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)ceil_div(n, 32)), dim3(256), 0, s,
+                           partials, g.nkchunks, npair_pad, g.cf * 16, g.npair, g.ncol, C);
+    }
     PBI4_HIP(hipGetLastError());
 }
 
-void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C, int npair,
-                        int ncol, int nb, const float *scale_exp, const double *shell_sum,
+void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C,
+                        const GemmPlan &g, const float *scale_exp, const double *shell_sum,
                         const double *nmodes, const float *counts, double kf3, double powSN,
                         double gridD, double sum_scale, double kcenter, double kbin, double *bout,
                         hipStream_t s) {
-    (void)npair;
     hipLaunchKernelGGL(k_bk_finalize, dim3((unsigned)ceil_div(itot, 128)), dim3(128), 0, s, rows,
-                       itot, tri, C, ncol, nb, scale_exp, shell_sum, nmodes, counts, kf3, powSN,
-                       gridD, sum_scale, kcenter, kbin, bout);
+                       itot, tri, C, g.cld, g.cs_l, g.cs_e, scale_exp, shell_sum, nmodes, counts,
+                       kf3, powSN, gridD, sum_scale, kcenter, kbin, bout);
     PBI4_HIP(hipGetLastError());
 }
 
-void launch_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, int ncol, int nb,
+void launch_bk_cross_finalize(int64_t itot, const int32_t *tri, const double *C, const GemmPlan &g,
                               const float *scale_exp, const float *counts, double kf3,
                               double gridD, double sum_scale, double kcenter, double kbin,
                               double *bout, hipStream_t s) {
     if (itot <= 0) return;
     hipLaunchKernelGGL(k_bk_cross_finalize, dim3((unsigned)ceil_div(itot, 256)), dim3(256), 0, s,
-                       itot, tri, C, ncol, nb, scale_exp, counts, kf3, gridD, sum_scale, kcenter,
-                       kbin, bout);
+                       itot, tri, C, g.cld, g.coff, g.nb, scale_exp, counts, kf3, gridD, sum_scale,
+                       kcenter, kbin, bout);
     PBI4_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,115 @@
+"""Block-sparse triangle GEMM against the dense one: every bispectrum entry bitwise equal.
+
+The block-sparse kernel multiplies, per wave of pair rows, only the column fragments its
+triangles read, with the same products, bf16 splits and MFMA sequence per K chunk as the dense
+kernel and the same FP64 split-K combine, so B (and B_2, B_4, and the cross bispectrum) must
+match bit for bit.  PBI4_TRIANGLE_GEMM=dense selects the dense kernel; =block_sparse makes the
+library refuse a shape that the block-sparse tiles would not cover, so a pass shows that the
+block-sparse kernel ran.  Its partial rows are never wider than the dense ones.
+"""
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import pbi4_oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pbi4_amd.core import Context
+    return Context(0)
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _field(ctx, N, n, seed, box=1000.0):
+    rng = np.random.default_rng(seed)
+    c = rng.random((3, 8))
+    k = rng.integers(0, 8, n)
+    p = (c[:, k] + 0.05 * rng.standard_normal((3, n))) % 1.0
+    p = np.minimum(p.astype(np.float32), np.float32(1.0 - 2 ** -24))
+    return ctx.density(ctx.assign(_dev(p), N, 4, True), N, 4, True, box, n)
+
+
+def _run(ctx, mode, fn, ws_fn):
+    old = os.environ.get("PBI4_TRIANGLE_GEMM")
+    try:
+        if mode is None:
+            os.environ.pop("PBI4_TRIANGLE_GEMM", None)
+        else:
+            os.environ["PBI4_TRIANGLE_GEMM"] = mode
+        ws = ws_fn()
+        out = fn().cpu().numpy()
+    finally:
+        if old is None:
+            os.environ.pop("PBI4_TRIANGLE_GEMM", None)
+        else:
+            os.environ["PBI4_TRIANGLE_GEMM"] = old
+    return ws, out
+
+
+# (grid, shells, kbin, kcenter, iopen, multipoles): the tile shapes of the real-space and
+# redshift-space workloads (16 x 3, 48 x 1, 48 x 3, 64 x 3) and small ones with one row fragment
+CASES = [
+    (64, 20, 1.0, 1.0, True, False),
+    (64, 16, 2.0, 2.0, True, True),
+    (64, 22, 1.0, 1.0, False, True),
+    (96, 48, 1.0, 1.0, True, False),
+    (96, 48, 1.0, 1.0, True, True),
+    (128, 64, 1.0, 1.0, True, True),
+]
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"N{c[0]}_nb{c[1]}_kb{c[2]:g}_open{int(c[4])}_ell{int(c[5])}")
+def test_block_sparse_equals_dense(ctx, case, prec):
+    N, nb, kb, kc, iopen, mult = case
+    n = 200000
+    print(f"block-sparse vs dense: N={N} nb={nb} kb={kb} iopen={iopen} multipoles={mult} prec={prec}",
+          flush=True)
+    delta = _field(ctx, N, n, 5 + nb)
+    itot = O.itot_count(nb, kb, kc, iopen)
+    ntr = ctx.count_triangles(nb, N, kb, kc, iopen, False, itot)
+    counts = _dev((ntr.astype(np.float64) * N ** 3).astype(np.float32))
+    kf = np.float32(2 * np.pi / 1000.0)
+    desc = ctx.bk_desc(nb, N, kb, kc, iopen, kf, itot, n, mult, (0.0, 0.0, 1.0), prec, 0)
+    ws_w, b_w = _run(ctx, "block_sparse", lambda: ctx.bk(delta, counts, desc),
+                     lambda: ctx.bk_workspace(desc))
+    ws_d, b_d = _run(ctx, "dense", lambda: ctx.bk(delta, counts, desc),
+                     lambda: ctx.bk_workspace(desc))
+    assert ws_w <= ws_d
+    assert np.all(np.isfinite(b_w))
+    bad = np.argwhere(b_w != b_d)
+    assert bad.size == 0, f"{len(bad)} entries differ, first {bad[:5].tolist()}"
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("case", [(64, 24, 1.0, 1.0, True), (96, 40, 1.0, 1.0, False)],
+                         ids=lambda c: f"N{c[0]}_nb{c[1]}_open{int(c[4])}")
+def test_block_sparse_cross_equals_dense(ctx, case, prec):
+    N, nb, kb, kc, iopen = case
+    n = 150000
+    print(f"block-sparse vs dense cross: N={N} nb={nb} iopen={iopen} prec={prec}", flush=True)
+    d1, d2 = _field(ctx, N, n, 31), _field(ctx, N, n, 32)
+    itot = O.itot_count(nb, kb, kc, iopen, cross=True)
+    ntr = ctx.count_triangles(nb, N, kb, kc, iopen, True, itot)
+    counts = _dev((ntr.astype(np.float64) * N ** 3).astype(np.float32))
+    kf = np.float32(2 * np.pi / 1000.0)
+    desc = ctx.bk_desc(nb, N, kb, kc, iopen, kf, itot, n, False, (0.0, 0.0, 1.0), prec, 0)
+    ws_w, b_w = _run(ctx, "block_sparse", lambda: ctx.bk_cross(d1, d2, counts, desc),
+                     lambda: ctx.bk_cross_workspace(desc))
+    ws_d, b_d = _run(ctx, "dense", lambda: ctx.bk_cross(d1, d2, counts, desc),
+                     lambda: ctx.bk_cross_workspace(desc))
+    assert ws_w <= ws_d
+    assert np.all(np.isfinite(b_w))
+    bad = np.argwhere(b_w != b_d)
+    assert bad.size == 0, f"{len(bad)} entries differ, first {bad[:5].tolist()}"
