@@ -42,6 +42,8 @@ typedef struct pbi4_ctx pbi4_ctx;
  * Replaces the module state of the f2py extensions imported at pbi4/pbi4.py:12-14. */
 int pbi4_ctx_create(int device, pbi4_ctx **out);
 int pbi4_ctx_destroy(pbi4_ctx *ctx);
+/* ctx = NULL: the message of the calling thread's last failed call made without a context
+ * (pbi4_assign_workspace, pbi4_triangle_sums_workspace return 0 when they refuse a shape). */
 const char *pbi4_last_error(const pbi4_ctx *ctx);
 int pbi4_version(void);
 

@@ -36,6 +36,21 @@ def kf3_f32(kf):
     return float(np.float32(np.float32(k * k) * k))
 
 
+def _assign_ws_bytes(desc, npart):
+    n = lib().pbi4_assign_workspace(ctypes.byref(desc), int(npart))
+    if n == 0:
+        raise Pbi4Error("pbi4_assign_workspace: " + lib().pbi4_last_error(None).decode())
+    return n
+
+
+def assign_workspace_bytes(grid, order, interlacing, npart):
+    """Workspace bytes pbi4_assign needs for these sizes (host only, no GPU).  Raises Pbi4Error
+    with the library's reason for a refused shape, e.g. a grid with no tile edge in [4, 16]."""
+    d = AssignDesc(int(grid), int(order), int(bool(interlacing)), _lib.POS_BOX, 1.0,
+                   (ctypes.c_float * 3)(0.0, 0.0, 0.0))
+    return _assign_ws_bytes(d, npart)
+
+
 class Context:
     """One pbi4_ctx (rocFFT plans and integer tables are cached in it)."""
 
@@ -71,10 +86,7 @@ class Context:
     def assign_workspace(self, grid, order, interlacing, npart):
         d = AssignDesc(int(grid), int(order), int(bool(interlacing)), _lib.POS_BOX, 1.0,
                        (ctypes.c_float * 3)(0.0, 0.0, 0.0))
-        n = lib().pbi4_assign_workspace(ctypes.byref(d), int(npart))
-        if n == 0:
-            raise Pbi4Error("pbi4_assign_workspace: invalid assignment descriptor")
-        return n
+        return _assign_ws_bytes(d, npart)
 
     def density_workspace(self, grid, order, interlacing, box, npart):
         d = DensityDesc(int(grid), int(order), int(bool(interlacing)), float(box), float(npart))
@@ -106,7 +118,7 @@ class Context:
         ng = 2 if interlacing else 1
         if out is None:
             out = torch.empty((ng, grid, grid, grid), dtype=torch.float32, device=self.device)
-        ws = self._ws_or(ws, lib().pbi4_assign_workspace(ctypes.byref(d), npart), "ws")
+        ws = self._ws_or(ws, _assign_ws_bytes(d, npart), "ws")
         self._check(lib().pbi4_assign(self.h, ctypes.byref(d), _ptr(pos), _ptr(vel), npart,
                                       _ptr(out), _ptr(ws), ws.numel(), _stream(stream)))
         return out
@@ -310,7 +322,10 @@ class Context:
         _check_dev(maps, "maps", torch.float32)
         Kstride = maps.shape[1]
         C = torch.empty((nb * (nb + 1) // 2, nb * nell), dtype=torch.float64, device=self.device)
-        ws = self._ws(lib().pbi4_triangle_sums_workspace(nb, nell, K, Kstride, precision))
+        n = lib().pbi4_triangle_sums_workspace(nb, nell, K, Kstride, precision)
+        if n == 0:
+            raise Pbi4Error("pbi4_triangle_sums_workspace: " + lib().pbi4_last_error(None).decode())
+        ws = self._ws(n)
         self._check(lib().pbi4_triangle_sums(self.h, nb, nell, K, Kstride, precision, _ptr(maps),
                                              _ptr(C), _ptr(ws), ws.numel(), _stream(stream)))
         return C

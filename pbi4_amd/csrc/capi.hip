@@ -153,9 +153,29 @@ struct pbi4_ctx {
 
 namespace {
 
+// message of the calling thread's last failed call made without a context
+thread_local std::string g_err_noctx;
+
 int set_err(pbi4_ctx *ctx, const Error &e) {
-    if (ctx) ctx->err = e.msg;
+    if (ctx)
+        ctx->err = e.msg;
+    else
+        g_err_noctx = e.msg;
     return e.code;
+}
+
+// Workspace size of a call made without a context: 0 with the reason kept for
+// pbi4_last_error(NULL) when the descriptor is refused.
+template <class F>
+size_t sized(F &&f) {
+    try {
+        return f();
+    } catch (const Error &e) {
+        g_err_noctx = e.msg;
+    } catch (const std::exception &e) {
+        g_err_noctx = e.what();
+    }
+    return 0;
 }
 
 template <class F>
@@ -791,14 +811,13 @@ int pbi4_ctx_destroy(pbi4_ctx *ctx) {
     return PBI4_OK;
 }
 
-const char *pbi4_last_error(const pbi4_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+const char *pbi4_last_error(const pbi4_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err_noctx.c_str(); }
 
 size_t pbi4_assign_workspace(const pbi4_assign_desc *d, int64_t npart) {
-    try {
+    return sized([&] {
+        PBI4_CHECK(d != nullptr, PBI4_EINVAL, "NULL descriptor");
         return assign_workspace(make_assign_plan(*d, npart), npart);
-    } catch (...) {
-        return 0;
-    }
+    });
 }
 
 int pbi4_assign(pbi4_ctx *ctx, const pbi4_assign_desc *d, const float *pos, const float *vel,
@@ -1181,12 +1200,10 @@ int pbi4_measure_pk_bk(pbi4_ctx *ctx, const pbi4_pk_desc *pd, const pbi4_bk_desc
 }
 
 size_t pbi4_triangle_sums_workspace(int nb, int nell, int64_t K, int64_t Kstride, int precision) {
-    try {
+    return sized([&] {
         GemmPlan g = make_gemm_plan(nb, nell, K, Kstride, precision);
         return align_up(gemm_partials_floats(g) * sizeof(float), 256) + 256;
-    } catch (...) {
-        return 0;
-    }
+    });
 }
 
 int pbi4_triangle_sums(pbi4_ctx *ctx, int nb, int nell, int64_t K, int64_t Kstride, int precision,

@@ -297,49 +297,44 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
         }
     }
     __syncthreads();
-    // partials are [quantity][work-group]: the finalize reads each quantity contiguously
+    // partials are [work-group][quantity]: one contiguous row per work-group
     for (int i = threadIdx.x; i < nq; i += blockDim.x) {
         float s = acc[i];
         for (int w = 1; w < kBinWaves; ++w) s += acc[w * nq + i];
-        partials[(int64_t)i * gridDim.x + blockIdx.x] = s;
+        partials[(int64_t)blockIdx.x * nq + i] = s;
     }
 }
 
-// FP64 combine of the per-work-group partials: one block per bin; thread t sums work-groups
-// t, t + 256, ... in that order, then a fixed pairwise tree over the threads (deterministic);
-// then pout / shell sums.
-constexpr int kFinThreads = 256;
+// FP64 sum of every quantity over the work-group rows: block b owns quantities 64 b .. 64 b + 63
+// (a wave reads 64 consecutive floats of a row), wave w sums rows w, w + 4, ... in that order,
+// then the four waves are added in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_bin_reduce(const float *partials, int nwg, int nq,
+                                                    double *qsum) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (q < nq)
+        for (int g = w; g < nwg; g += 4) s += (double)partials[(int64_t)g * nq + q];
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && q < nq) qsum[q] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
 
-__global__ __launch_bounds__(kFinThreads) void k_bin_finalize(BinPlan b, const float *partials,
-                                                              double kF3, double *pk_out,
-                                                              double *shell_sum) {
-    __shared__ double red[5][kFinThreads];
-    const int i = blockIdx.x;
+// pout / shell sums from the FP64 quantity sums: one thread per bin.
+__global__ void k_bin_finalize(BinPlan b, const double *qsum, double kF3, double *pk_out,
+                               double *shell_sum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.nbins_pk + b.nshell) return;
     const bool pk = i < b.nbins_pk;
     const int nv = pk ? 5 : 1;
     // quantity q of bin i: column q * nbins_pk + i (P(k) bins) or 4 nbins_pk + i (shells)
     const int col0 = pk ? i : 4 * b.nbins_pk + i;
     const int cstride = pk ? b.nbins_pk : 0;
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int g = threadIdx.x; g < b.nwg; g += kFinThreads) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q)
-            if (q < nv) s[q] += (double)partials[(int64_t)(col0 + q * cstride) * b.nwg + g];
-    }
-#pragma unroll
-    for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = s[q];
-    __syncthreads();
-    for (int w = kFinThreads / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-#pragma unroll
-            for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) s[q] = red[q][0];
+    for (int q = 0; q < 5; ++q)
+        if (q < nv) s[q] = qsum[col0 + q * cstride];
     if (pk) {
         double *p = pk_out + 5 * (int64_t)i;
         const double co = s[0];
@@ -719,8 +714,13 @@ void launch_combine(int N, int interlacing, const float *wtab, const float2 *ph,
     PBI4_HIP(hipGetLastError());
 }
 
+// per-work-group rows of FP32 partials, then the FP64 sum of each quantity
+static size_t bin_rows_floats(const BinPlan &b) {
+    return align_up((size_t)b.nwg * (5 * b.nbins_pk + b.nshell), 2);
+}
+
 size_t bin_partials_floats(const BinPlan &b) {
-    return (size_t)b.nwg * (5 * b.nbins_pk + b.nshell);
+    return bin_rows_floats(b) + 2 * (size_t)(5 * b.nbins_pk + b.nshell);
 }
 
 void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStream_t s) {
@@ -744,8 +744,13 @@ void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, do
                          double *shell_sum, hipStream_t s) {
     const int n = b.nbins_pk + b.nshell;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)n), dim3(kFinThreads), 0, s, b, partials, kF3,
-                       pk_out, shell_sum);
+    const int nq = 5 * b.nbins_pk + b.nshell;
+    // the FP64 sums follow the rows in the same buffer (8-byte aligned: rows are an even count)
+    double *qsum = reinterpret_cast<double *>(const_cast<float *>(partials) + bin_rows_floats(b));
+    hipLaunchKernelGGL(k_bin_reduce, dim3((unsigned)ceil_div(nq, 64)), dim3(256), 0, s, partials,
+                       b.nwg, nq, qsum);
+    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, 64)), dim3(64), 0, s, b, qsum,
+                       kF3, pk_out, shell_sum);
     PBI4_HIP(hipGetLastError());
 }
 

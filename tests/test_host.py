@@ -40,6 +40,17 @@ def test_library_errors_are_status_codes_without_gpu():
     assert L.pbi4_assign_workspace(ctypes.byref(d), 100) > 100 * 12
 
 
+@pytest.mark.parametrize("grid", [34, 38, 46, 58, 62, 74])
+def test_grids_without_tile_edge_are_refused_with_the_reason(grid):
+    """The deposit tiles the grid with edges in [4, 16]; a grid 2p with p a prime above 16 has
+    none and is refused before any device work, with a message that says so."""
+    from pbi4_amd.core import assign_workspace_bytes
+    from pbi4_amd._lib import Pbi4Error
+    with pytest.raises(Pbi4Error, match=rf"grid {grid} has no divisor in \[4, 16\]"):
+        assign_workspace_bytes(grid, 4, True, 1000)
+    assert assign_workspace_bytes(grid + 2, 4, True, 1000) > 1000 * 12   # 36, 40, 48, 60, 64, 76
+
+
 def _config(tmp_path, **kw):
     cfg = {"cat": "npz", "box": 1000, "grid": 64.0, "interpolation": 4, "interlacing": True,
            "kbin": 1.0, "kcenter": 1.0, "iopen": True, "om": 0.3, "ol": 0.7, "z": 0.5,
