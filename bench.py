@@ -386,7 +386,8 @@ def main():
     del eng
 
     # ---------------------------------------------------------------- cfg4 stream
-    guarded("cfg4", lambda: cfg4_stream(ctx, torch, args, remaining(args) - 170), out, args, 120)
+    if args.cfg4_requests:
+        guarded("cfg4", lambda: cfg4_stream(ctx, torch, args, remaining(args) - 170), out, args, 120)
 
     # ---------------------------------------------------------------- precision + CPU baseline
     cpu = None

@@ -305,27 +305,42 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinPlan b, const float2 *de
     }
 }
 
-// FP64 sum of every quantity over the work-group rows: block b owns quantities 64 b .. 64 b + 63
-// (a wave reads 64 consecutive floats of a row), wave w sums rows w, w + 4, ... in that order,
-// then the four waves are added in a fixed order (deterministic).
+// FP64 sums of every quantity over the work-group rows, in kBinSegs segments: block (b, seg) owns
+// quantities 64 b .. 64 b + 63 (a wave reads 64 consecutive floats of a row) and rows
+// seg + kBinSegs (w + 4 r), r = 0, 1, ... (wave w), loads issued 8 at a time and added in row order;
+// the four waves are added in a fixed order into qseg[seg][q] (deterministic).
+constexpr int kBinSegs = 16;
+
 __global__ __launch_bounds__(256) void k_bin_reduce(const float *partials, int nwg, int nq,
-                                                    double *qsum) {
+                                                    double *qseg) {
     __shared__ double red[4][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, seg = blockIdx.y;
     const int q = blockIdx.x * 64 + lane;
+    const int step = 4 * kBinSegs;
     double s = 0.0;
-    if (q < nq)
-        for (int g = w; g < nwg; g += 4) s += (double)partials[(int64_t)g * nq + q];
+    if (q < nq) {
+        int g = seg + kBinSegs * w;
+        for (; g + 7 * step < nwg; g += 8 * step) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partials[(int64_t)(g + u * step) * nq + q];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += (double)v[u];
+        }
+        for (; g < nwg; g += step) s += (double)partials[(int64_t)g * nq + q];
+    }
     red[w][lane] = s;
     __syncthreads();
-    if (w == 0 && q < nq) qsum[q] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (w == 0 && q < nq)
+        qseg[(int64_t)seg * nq + q] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
-// pout / shell sums from the FP64 quantity sums: one thread per bin.
-__global__ void k_bin_finalize(BinPlan b, const double *qsum, double kF3, double *pk_out,
+// pout / shell sums from the segment sums (added in segment order): one thread per bin.
+__global__ void k_bin_finalize(BinPlan b, const double *qseg, double kF3, double *pk_out,
                                double *shell_sum) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.nbins_pk + b.nshell) return;
+    const int nq = 5 * b.nbins_pk + b.nshell;
     const bool pk = i < b.nbins_pk;
     const int nv = pk ? 5 : 1;
     // quantity q of bin i: column q * nbins_pk + i (P(k) bins) or 4 nbins_pk + i (shells)
@@ -334,7 +349,8 @@ __global__ void k_bin_finalize(BinPlan b, const double *qsum, double kF3, double
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 5; ++q)
-        if (q < nv) s[q] = qsum[col0 + q * cstride];
+        if (q < nv)
+            for (int seg = 0; seg < kBinSegs; ++seg) s[q] += qseg[(int64_t)seg * nq + col0 + q * cstride];
     if (pk) {
         double *p = pk_out + 5 * (int64_t)i;
         const double co = s[0];
@@ -720,7 +736,7 @@ static size_t bin_rows_floats(const BinPlan &b) {
 }
 
 size_t bin_partials_floats(const BinPlan &b) {
-    return bin_rows_floats(b) + 2 * (size_t)(5 * b.nbins_pk + b.nshell);
+    return bin_rows_floats(b) + 2 * (size_t)kBinSegs * (5 * b.nbins_pk + b.nshell);
 }
 
 void launch_bin(const BinPlan &b, const float2 *delta, float *partials, hipStream_t s) {
@@ -745,11 +761,12 @@ void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, do
     const int n = b.nbins_pk + b.nshell;
     if (n <= 0) return;
     const int nq = 5 * b.nbins_pk + b.nshell;
-    // the FP64 sums follow the rows in the same buffer (8-byte aligned: rows are an even count)
-    double *qsum = reinterpret_cast<double *>(const_cast<float *>(partials) + bin_rows_floats(b));
-    hipLaunchKernelGGL(k_bin_reduce, dim3((unsigned)ceil_div(nq, 64)), dim3(256), 0, s, partials,
-                       b.nwg, nq, qsum);
-    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, 64)), dim3(64), 0, s, b, qsum,
+    // the FP64 segment sums follow the rows in the same buffer (8-byte aligned: the rows are an
+    // even count of floats)
+    double *qseg = reinterpret_cast<double *>(const_cast<float *>(partials) + bin_rows_floats(b));
+    hipLaunchKernelGGL(k_bin_reduce, dim3((unsigned)ceil_div(nq, 64), kBinSegs), dim3(256), 0, s,
+                       partials, b.nwg, nq, qseg);
+    hipLaunchKernelGGL(k_bin_finalize, dim3((unsigned)ceil_div(n, 64)), dim3(64), 0, s, b, qseg,
                        kF3, pk_out, shell_sum);
     PBI4_HIP(hipGetLastError());
 }
