@@ -60,7 +60,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=6,
+                    help="concurrent slots of the headline engine (measured 4: 391.6, 6: 405.2, "
+                         "8: 405.1 catalogues/s)")
     ap.add_argument("--pool", type=int, default=4, help="distinct resident catalogues")
     ap.add_argument("--lattice", type=int, default=256)
     ap.add_argument("--grid", type=int, default=256)
@@ -74,6 +76,7 @@ def parse():
     ap.add_argument("--stages", type=int, default=1)
     ap.add_argument("--cfg4-requests", type=int, default=2000)
     ap.add_argument("--cfg4-pool", type=int, default=24)
+    ap.add_argument("--cfg4-slots", type=int, default=4)
     ap.add_argument("--cfg4-fp32", type=int, default=100,
                     help="requests of the cfg4 stream timed through the plain FP32 path")
     ap.add_argument("--cfg3", type=int, default=1)
@@ -155,33 +158,58 @@ def triangle_list(nb, kbin, kcenter, iopen):
             for i in range(max(1, l - j + 1 - ishift), j + 1)]
 
 
-def gemm_blocks(nb, nell, kbin, kcenter, iopen):
+def gemm_blocks(nb, nell, kbin, kcenter, iopen, pair_rows=False):
     """16-row x 16-column MFMA tiles the triangle GEMM issues, dense and block-sparse, and the
-    number of sums the finalize reads.  Block-sparse (the library's rule): columns in (l, ell)
-    order; a wave of 16 rf pair rows spans the fragments between the first and last column its
-    triangles read, and all 4 waves of a work-group multiply the widest span of the four (the
-    dense kernel where that would be more than 80 % of the dense tiles)."""
+    number of sums the finalize reads.  Block-sparse (the library's rule): a wave of 16 rf GEMM
+    rows spans the column fragments between the first and last column its triangles read, and
+    all 4 waves of a work-group multiply the widest span of the four.  Real space (nell = 1)
+    takes the rows (j, l) of the triangle loop, l-major from j = max(1, ceil((l + 1 - ishift) /
+    2)), with the leg i as the column; redshift space the pairs (i <= j) with columns (l, ell),
+    kept dense where the spans would be more than 80 % of the dense tiles.  pair_rows: the pair
+    rows in real space too, without the 80 % rule (PBI4_TRIANGLE_GEMM=block_sparse)."""
+    ishift = int(kcenter / kbin) + (1 if iopen else 0)
+
+    def rf_of(rows, cf):
+        rf = 4 if rows >= 900 else 2 if rows >= 300 else 1
+        if cf >= 12 and rf > 2:
+            rf = 2
+        if rows <= 256 and cf >= 3:
+            rf = 4
+        return rf
+
     npair = nb * (nb + 1) // 2
     ncol = nb * nell
     cf = next(c for c in (1, 2, 3, 4, 6, 8, 9, 12) if c >= -(-ncol // 16))
-    rf = 4 if npair >= 900 else 2 if npair >= 300 else 1
-    if cf >= 12 and rf > 2:
-        rf = 2
-    if npair <= 256 and cf >= 3:
-        rf = 4
-    nblocks = -(-npair // (64 * rf))
-    lo, hi = [1 << 30] * (4 * nblocks), [-1] * (4 * nblocks)
+    rf = rf_of(npair, cf)
+    dense = 4 * (-(-npair // (64 * rf))) * rf * cf
     tl = triangle_list(nb, kbin, kcenter, iopen)
+    if nell == 1 and not pair_rows:
+        j0 = [0] + [max(1, (l + 2 - ishift) // 2) for l in range(1, nb + 1)]
+        off = [0, 0]
+        for l in range(1, nb + 1):
+            off.append(off[-1] + l - j0[l] + 1)
+        rows = off[-1]
+        rf = rf_of(rows, cf)
+
+        def entry(l, j, i):
+            return off[l] + j - j0[l], i - 1, i - 1
+    else:
+        rows = npair
+
+        def entry(l, j, i):
+            return (j - 1) * j // 2 + (i - 1), (l - 1) * nell, (l - 1) * nell + nell - 1
+    nblocks = -(-rows // (64 * rf))
+    lo, hi = [1 << 30] * (4 * nblocks), [-1] * (4 * nblocks)
     for l, j, i in tl:
-        w = ((j - 1) * j // 2 + (i - 1)) // (16 * rf)
-        lo[w] = min(lo[w], (l - 1) * nell // 16)
-        hi[w] = max(hi[w], ((l - 1) * nell + nell - 1) // 16)
+        r, c0, c1 = entry(l, j, i)
+        w = r // (16 * rf)
+        lo[w] = min(lo[w], c0 // 16)
+        hi[w] = max(hi[w], c1 // 16)
     sparse = 0
     for b in range(nblocks):
         spans = [hi[w] - lo[w] + 1 for w in range(4 * b, 4 * b + 4) if hi[w] >= 0]
         sparse += 4 * rf * max(spans, default=0)
-    dense = 4 * nblocks * rf * cf
-    if 5 * sparse > 4 * dense:      # the library keeps the dense kernel above 80 % of its tiles
+    if nell > 1 and not pair_rows and 5 * sparse > 4 * dense:   # the library keeps dense there
         sparse = dense
     return {"dense": dense, "block_sparse": sparse, "useful": len(tl) * nell}
 
@@ -485,7 +513,7 @@ def cfg4_stream(ctx, torch, args, budget):
                 for k in ids]
 
     t_start = time.perf_counter()
-    eng = ServingEngine(ctx, slots=args.slots)
+    eng = ServingEngine(ctx, slots=args.cfg4_slots)
     first = {}
     for k, (c, g, o) in enumerate(draws):
         first.setdefault((g, o), k)
@@ -844,9 +872,11 @@ def gemm_bench(ctx, torch):
       all_sums:            every (pair, column) sum, through pbi4_triangle_sums on random
                            unit-variance maps (dense kernel);
       in_bispectrum:       the bispectrum's own GEMM (block-sparse: the column windows its
-                           triangles read), timed inside pbi4_measure_bk on a random 512^3 half
-                           spectrum;
-      in_bispectrum_dense: the same with PBI4_TRIANGLE_GEMM=dense.
+                           triangles read; real space on the (j, l) rows), timed inside
+                           pbi4_measure_bk on a random 512^3 half spectrum;
+      in_bispectrum_dense: the same with PBI4_TRIANGLE_GEMM=dense;
+      in_bispectrum_pair_rows (real space): PBI4_TRIANGLE_GEMM=block_sparse, the (i <= j) pair
+                           rows.
     executed_tflops counts the MFMA work issued (x3 for split-BF16), useful_tflops only the sums
     that are read; mfma_frac = executed / peak."""
     from pbi4_amd import _lib
@@ -878,6 +908,11 @@ def gemm_bench(ctx, torch):
         maps = torch.randn((nb * nell, Ks), generator=g, device="cuda", dtype=torch.float32)
         maps[:, K:] = 0
         tiles = gemm_blocks(nb, nell, 1.0, 1.0, True)
+        pair_tiles = gemm_blocks(nb, nell, 1.0, 1.0, True, pair_rows=True)["block_sparse"]
+        modes = [("dense", "_in_bispectrum_dense", tiles["dense"]),
+                 (None, "_in_bispectrum", tiles["block_sparse"])]
+        if nell == 1:
+            modes.append(("block_sparse", "_in_bispectrum_pair_rows", pair_tiles))
         res = {}
         for prec, name, peak, x in ((_lib.PREC_BF16X3, "bf16x3", BF16_PEAK_TFLOPS, 3),
                                     (_lib.PREC_FP32, "fp32", 157.3, 1)):
@@ -892,7 +927,7 @@ def gemm_bench(ctx, torch):
             desc = ctx.bk_desc(nb, N, 1.0, 1.0, True, kf, itot, 10 ** 8, nell == 3,
                                (0.0, 0.0, 1.0), prec, 0)
             assert ctx.maps_grid(desc) == Np
-            for mode, key in (("dense", "_in_bispectrum_dense"), (None, "_in_bispectrum")):
+            for mode, key, issued in modes:
                 if mode:
                     os.environ["PBI4_TRIANGLE_GEMM"] = mode
                 try:
@@ -900,7 +935,7 @@ def gemm_bench(ctx, torch):
                     avg = timed(lambda: ctx.bk(delta, counts, desc, ws=ws))
                 finally:
                     os.environ.pop("PBI4_TRIANGLE_GEMM", None)
-                res[name + key] = rates(avg, tiles["dense"] if mode else tiles["block_sparse"])
+                res[name + key] = rates(avg, issued)
                 del ws
             res[name + "_block_sparse_speedup"] = round(
                 res[name + "_in_bispectrum_dense"]["avg_ms"] / res[name + "_in_bispectrum"]["avg_ms"], 3)

@@ -567,8 +567,10 @@ int choose_maps_grid(int N, int R) {
     return N;
 }
 
+int tri_ishift(float kbin, float kcenter, int iopen) { return (int)(kcenter / kbin) + (iopen ? 1 : 0); }
+
 std::vector<int32_t> triangle_list(int nb, float kbin, float kcenter, int iopen, int cross) {
-    const int ishift = (int)(kcenter / kbin) + (iopen ? 1 : 0);
+    const int ishift = tri_ishift(kbin, kcenter, iopen);
     std::vector<int32_t> t;
     for (int l = 1; l <= nb; ++l)
         for (int j = 1; j <= l; ++j) {
@@ -670,7 +672,8 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a, int nbins_pk 
     L.K = (int64_t)L.Np * L.Np * L.Np;
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, L.nell, L.K, L.Kstride, d.precision);
-    gemm_set_windows(L.g, triangle_list(d.nbins, d.kbin, d.kcenter, d.iopen, 0), false);
+    gemm_set_windows(L.g, triangle_list(d.nbins, d.kbin, d.kcenter, d.iopen, 0), false,
+                     tri_ishift(d.kbin, d.kcenter, d.iopen));
     BinPlan b = make_bin_plan(d.grid, nbins_pk, d.nbins, 0, d.los, 1.0f);
     const size_t work = maps_work(ctx, sh, L.Np, d.grid, L.nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));
@@ -756,7 +759,8 @@ BkCrossLayout bk_cross_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     L.K = (int64_t)L.Np * L.Np * L.Np;
     L.Kstride = (int64_t)align_up((size_t)L.K, 64);
     L.g = make_gemm_plan(d.nbins, 2, L.K, L.Kstride, d.precision);
-    gemm_set_windows(L.g, triangle_list(d.nbins, d.kbin, d.kcenter, d.iopen, 1), true);
+    gemm_set_windows(L.g, triangle_list(d.nbins, d.kbin, d.kcenter, d.iopen, 1), true,
+                     tri_ishift(d.kbin, d.kcenter, d.iopen));
     BinPlan b = make_bin_plan(d.grid, 0, d.nbins, 0, d.los, 1.0f);
     const size_t work = maps_work(ctx, sh, L.Np, d.grid, nmaps, L.Kstride);
     L.bin_partials = a.take<float>(bin_partials_floats(b));

@@ -139,16 +139,22 @@ struct GemmPlan {
     int maxw = 0;       // widest window (column fragments): partial rows are 16 maxw long
     int ucap = 0;       // most column fragments one work-group stages (union of its 4 waves)
     int moff = 0, mdiv = 1, ncol_l = 0;
+    // rows: 0 = pairs (i <= j) at j (j + 1) / 2 + i; 1 = real-space rows (j, l) of the triangle
+    // loop, ordered by l then j from max(1, ceil((l + 1 - ish) / 2)), columns = the leg i
+    int rowmode = 0, ish = 0;
     std::vector<uint8_t> wstart, wwidth;
     // finalize indexing: auto triangle (l, ell) reads C column (l - 1) * cs_l + ell * cs_e; cross
     // triangle i reads column coff + i - 1; cld is the C row length
     int cs_l = 1, cs_e = 0, coff = 0, cld = 0;
 };
-GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision);
+// nrows > 0: that many GEMM rows instead of the nb (nb + 1) / 2 pairs (block-sparse row forms)
+GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision,
+                        int nrows = 0);
 // Switch plan g to block-sparse mode for the triangles in tri ((l, j, i) triples, 1-based, as
-// triangle_list writes them); cross: the cross-bispectrum layout (pair rows (j <= l) of field 1,
-// field-2 map i).  Keeps the dense kernel where the windows do not fit the compiled tiles.
-void gemm_set_windows(GemmPlan &g, const std::vector<int32_t> &tri, bool cross);
+// triangle_list writes them, ishift = int(kcenter / kbin) + iopen); cross: the cross-bispectrum
+// layout (pair rows (j <= l) of field 1, field-2 map i); real space (nell = 1) uses the (j, l)
+// rows.  Keeps the dense kernel where the windows do not fit the compiled tiles.
+void gemm_set_windows(GemmPlan &g, const std::vector<int32_t> &tri, bool cross, int ishift);
 size_t gemm_partials_floats(const GemmPlan &g);
 void launch_gemm(const GemmPlan &g, const float *maps, float *partials, hipStream_t s,
                  KernelTimer *timer);

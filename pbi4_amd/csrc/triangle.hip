@@ -37,6 +37,35 @@ __device__ __forceinline__ void pair_of(int p, int &i, int &j) {
     i = p - jj * (jj + 1) / 2;
 }
 
+// First row (1-based j) of l in the real-space row order: j >= max(1, ceil((l + 1 - ishift) / 2)),
+// the j of the triangle loop that have an i with max(1, l - j + 1 - ishift) <= i <= j.
+__host__ __device__ __forceinline__ int jl_first(int l, int ish) {
+    const int j = (l + 2 - ish) / 2;
+    return j > 1 ? j : 1;
+}
+
+// Maps (a <= b, 0-based) whose product is GEMM row p.  Mode 0: the pair (i <= j) at
+// p = j (j + 1) / 2 + i (rows of the dense kernel and of the multipole form).  Mode 1 (real
+// space): the rows (j, l) of the triangle loop in (l, j) order, j from jl_first(l) to l, whose
+// column is the smallest leg i; rows past the last are padding (maps 0, 0).
+__device__ __forceinline__ void row_pair(int p, int mode, int nb, int ish, int &a, int &b) {
+    if (mode == 0) {
+        pair_of(p, a, b);
+        return;
+    }
+    a = b = 0;
+    int off = 0;
+    for (int l = 1; l <= nb; ++l) {
+        const int j0 = jl_first(l, ish), n = l - j0 + 1;
+        if (p < off + n) {
+            a = j0 + (p - off) - 1;
+            b = l - 1;
+            return;
+        }
+        off += n;
+    }
+}
+
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
@@ -272,7 +301,8 @@ template <int PREC, int W, int NVB, int RF>
 __device__ __forceinline__ void win_body(const float *maps, int64_t K, int64_t Kstride, int nb,
                                          int moff, int mdiv, int ncol, int npair, int kchunk,
                                          int nrowblocks, int rowblock, int chunk, int ucap, int nw,
-                                         const GemmWindows &win, float *partials) {
+                                         int rowmode, int ish, const GemmWindows &win,
+                                         float *partials) {
     constexpr int KT = kWinKT, LDF = KT + kPadF, LDB = KT + kPadB;
     constexpr int NVA = kWinMaxMaps * (KT / 4) / kGemmThreads;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -292,16 +322,16 @@ __device__ __forceinline__ void win_body(const float *maps, int64_t K, int64_t K
     const int t0 = win.start[wid] - u0;         // the wave's first fragment in the staged tile
     const int64_t kbeg = (int64_t)chunk * kchunk;
     const int64_t kend = min(K, kbeg + kchunk);
-    int ilast, jlast;
-    pair_of(min(npair, (rowblock + 1) * 4 * RF * 16) - 1, ilast, jlast);
-    const int amax = jlast + 1;                 // pair rows (i <= j) read maps 0 .. jlast
+    int alast, blast;
+    row_pair(min(npair, (rowblock + 1) * 4 * RF * 16) - 1, rowmode, nb, ish, alast, blast);
+    const int amax = blast + 1;                 // rows (a <= b, b non-decreasing) read maps 0 .. blast
 
     int pi[RF], pj[RF];
 #pragma unroll
     for (int r = 0; r < RF; ++r) {
         const int prow = wid * RF * 16 + r * 16 + (lane & 15);
         pi[r] = pj[r] = 0;
-        if (prow < npair) pair_of(prow, pi[r], pj[r]);
+        if (prow < npair) row_pair(prow, rowmode, nb, ish, pi[r], pj[r]);
     }
     f32x4 acc[RF][W];
 #pragma unroll
@@ -460,7 +490,8 @@ __device__ __forceinline__ void win_body(const float *maps, int64_t K, int64_t K
 template <int PREC, int CFW, int RF>
 __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm_win(
     const float *maps, int64_t K, int64_t Kstride, int nb, int moff, int mdiv, int ncol, int npair,
-    int kchunk, int nrowblocks, int nchunks, int ucap, int nw, GemmWindows win, float *partials) {
+    int kchunk, int nrowblocks, int nchunks, int ucap, int nw, int rowmode, int ish, GemmWindows win,
+    float *partials) {
     constexpr int NVB = WinTile<CFW>::NVB;
     const int w = blockIdx.x;
     const int q = w / kXcds;
@@ -472,7 +503,8 @@ __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm_win(
     case n:                                                                                    \
         if constexpr (n <= CFW)                                                                \
             win_body<PREC, n, NVB, RF>(maps, K, Kstride, nb, moff, mdiv, ncol, npair, kchunk,  \
-                                       nrowblocks, rowblock, chunk, ucap, nw, win, partials);  \
+                                       nrowblocks, rowblock, chunk, ucap, nw, rowmode, ish, win, \
+                                       partials);                                             \
         break;
     switch (W) {
         PBI4_WIN_CASE(1)
@@ -564,7 +596,8 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const float *partials, int 
 // One thread per triangle m (reference order): bout(:, m) of bispectrumI4.f90:173-182 / :489-500
 // with pow(s) from Parseval and the triple sums unscaled from the band-limited maps grid.
 __global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double *C,
-                              int cld, int cs_l, int cs_e, const float *scale_exp,
+                              int cld, int cs_l, int cs_e, int rowmode, int ish,
+                              const float *scale_exp,
                               const double *shell_sum,
                               const double *nmodes, const float *counts, double kf3, double powSN,
                               double gridD, double sum_scale, double kcenter, double kbin,
@@ -576,12 +609,20 @@ __global__ void k_bk_finalize(int rows, int64_t itot, const int32_t *tri, const 
     const double pl = pw(l), pj = pw(j), pi_ = pw(i);
     const double powSN2 = powSN * powSN;
     const double bispSN = ((pi_ + pj) + pl) * powSN + powSN2;
-    const int64_t p = (int64_t)(j - 1) * j / 2 + (i - 1);
+    // C entry of the triangle: row (i <= j), column (l, ell); or (real-space rows) row (j, l),
+    // column i
+    const double *cp;
+    if (rowmode == 0) {
+        cp = C + ((int64_t)(j - 1) * j / 2 + (i - 1)) * cld + (int64_t)(l - 1) * cs_l;
+    } else {
+        int64_t row = j - jl_first(l, ish);
+        for (int ll = 1; ll < l; ++ll) row += ll - jl_first(ll, ish) + 1;
+        cp = C + row * cld + (i - 1);
+    }
     const double unscale =
         sum_scale * ldexp(1.0, -(int)(scale_exp[i - 1] + scale_exp[j - 1] + scale_exp[l - 1]));
     const double cnt = (double)counts[m];
     double *b = bout + (int64_t)rows * m;
-    const double *cp = C + p * cld + (int64_t)(l - 1) * cs_l;
     const double sum = cp[0] * unscale;
     b[0] = kcenter + (double)(l - 1) * kbin;
     b[1] = kcenter + (double)(j - 1) * kbin;
@@ -674,12 +715,12 @@ static int gemm_cf(int ncol) {
     return -1;
 }
 
-GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision) {
+GemmPlan make_gemm_plan(int nb, int nell, int64_t K, int64_t Kstride, int precision, int nrows) {
     GemmPlan g{};
     g.nb = nb;
     g.nell = nell;
     g.ncol = nb * nell;
-    g.npair = nb * (nb + 1) / 2;
+    g.npair = nrows > 0 ? nrows : nb * (nb + 1) / 2;
     g.K = K;
     g.Kstride = Kstride;
     g.precision = precision;
@@ -721,41 +762,19 @@ size_t gemm_partials_floats(const GemmPlan &g) {
     return (size_t)g.nkchunks * g.nrowblocks * 64 * g.rf * (g.win ? g.maxw : g.cf) * 16;
 }
 
-void gemm_set_windows(GemmPlan &g, const std::vector<int32_t> &tri, bool cross) {
-    g.win = false;
-    g.ncol_l = cross ? g.nb : g.nb * g.nell;
-    g.moff = cross ? g.nb : 0;
-    g.mdiv = cross ? 1 : g.nell;
-    // dense indexing (maps in ell-major order; the cross field-2 maps after the field-1 maps)
-    g.cld = g.ncol;
-    g.cs_l = 1;
-    g.cs_e = g.nb;
-    g.coff = cross ? g.nb : 0;
-    // PBI4_TRIANGLE_GEMM=dense keeps the dense kernel, =block_sparse refuses shapes that the
-    // block-sparse tiles do not cover (comparisons and timing of the two)
-    const char *mode = std::getenv("PBI4_TRIANGLE_GEMM");
-    if (mode && std::strcmp(mode, "dense") == 0) return;
-    const bool force = mode && std::strcmp(mode, "block_sparse") == 0;
-    auto dense = [&](const char *why) {
-        PBI4_CHECK(!force, PBI4_EUNSUPPORTED,
-                   std::string("PBI4_TRIANGLE_GEMM=block_sparse: ") + why);
-    };
+// Per-wave windows of plan g for the C entries (row, logical column) of the triangles; false
+// (with the reason) if they do not fit the compiled block-sparse tiles.
+template <class Entry>
+static bool plan_windows(GemmPlan &g, const std::vector<int32_t> &tri, const Entry &entry,
+                         std::string &why) {
     const int nwaves = g.nrowblocks * 4, wave_rows = 16 * g.rf;
-    if (g.nb > kWinMaxMaps) return dense("more than 64 shells");
-    if (nwaves > kMaxWinWaves) return dense("too many pair rows");
+    if (g.nb > kWinMaxMaps) return why = "more than 64 shells", false;
+    if (nwaves > kMaxWinWaves) return why = "too many pair rows", false;
     std::vector<int> lo(nwaves, 1 << 30), hi(nwaves, -1);
     for (size_t m = 0; m + 2 < tri.size(); m += 3) {
-        const int64_t l = tri[m], j = tri[m + 1], i = tri[m + 2];
-        int64_t p, c0, c1;
-        if (cross) {            // pair (j <= l) of field 1, field-2 column i
-            p = (l - 1) * l / 2 + (j - 1);
-            c0 = c1 = i - 1;
-        } else {                // pair (i <= j), columns (l, ell) for every ell
-            p = (j - 1) * j / 2 + (i - 1);
-            c0 = (l - 1) * g.nell;
-            c1 = c0 + g.nell - 1;
-        }
-        const int w = (int)(p / wave_rows);
+        int64_t row, c0, c1;
+        entry(tri[m], tri[m + 1], tri[m + 2], row, c0, c1);
+        const int w = (int)(row / wave_rows);
         lo[w] = std::min<int>(lo[w], (int)(c0 / 16));
         hi[w] = std::max<int>(hi[w], (int)(c1 / 16));
     }
@@ -790,29 +809,95 @@ void gemm_set_windows(GemmPlan &g, const std::vector<int32_t> &tri, bool cross) 
             cfw = c;
             break;
         }
-    if (maxw == 0) return dense("no triangles");
-    if (cfw == 0) return dense("a window wider than 8 column tiles");
+    if (maxw == 0) return why = "no triangles", false;
+    if (cfw == 0) return why = "a window wider than 8 column tiles", false;
     const int ucap_max = cfw == 2   ? WinTile<2>::UCAP
                          : cfw == 3 ? WinTile<3>::UCAP
                          : cfw == 4 ? WinTile<4>::UCAP
                          : cfw == 6 ? WinTile<6>::UCAP
                                     : WinTile<8>::UCAP;
-    if (ucap > ucap_max) return dense("a work-group's windows span too many column tiles");
-    // a work-group with every column tile gains nothing over the dense kernel's wider x-stages
-    // (64 x-values for up to 6 tiles) and single staging of each map
-    int64_t tiles = 0;
-    for (int b = 0; b < g.nrowblocks; ++b) tiles += width[b];
-    if (!force && 5 * tiles > 4 * (int64_t)g.nrowblocks * g.cf) return;
-    g.win = true;
+    if (ucap > ucap_max) return why = "a work-group's windows span too many column tiles", false;
     g.cfw = cfw;
     g.maxw = maxw;
     g.ucap = ucap;
     g.wstart = std::move(start);
     g.wwidth = std::move(width);
-    g.cld = g.ncol_l;
-    g.cs_l = cross ? 1 : g.nell;
-    g.cs_e = 1;
-    g.coff = 0;
+    return true;
+}
+
+void gemm_set_windows(GemmPlan &g, const std::vector<int32_t> &tri, bool cross, int ishift) {
+    g.win = false;
+    g.rowmode = 0;
+    g.ish = ishift;
+    g.ncol_l = cross ? g.nb : g.nb * g.nell;
+    g.moff = cross ? g.nb : 0;
+    g.mdiv = cross ? 1 : g.nell;
+    // dense indexing (maps in ell-major order; the cross field-2 maps after the field-1 maps)
+    g.cld = g.ncol;
+    g.cs_l = 1;
+    g.cs_e = g.nb;
+    g.coff = cross ? g.nb : 0;
+    // PBI4_TRIANGLE_GEMM=dense keeps the dense kernel, =block_sparse the block-sparse kernel on
+    // (i <= j) rows and refuses shapes its tiles do not cover (comparisons and timing)
+    const char *mode = std::getenv("PBI4_TRIANGLE_GEMM");
+    if (mode && std::strcmp(mode, "dense") == 0) return;
+    const bool force = mode && std::strcmp(mode, "block_sparse") == 0;
+    std::string why;
+    // real space: rows (j, l) of the triangle loop with the smallest leg i as the column -- about
+    // half the (i <= j) pair rows (624 of 1176 at 48 shells of k_F) and the forming of the pair
+    // products is what bounds the block-sparse kernel
+    if (!cross && g.nell == 1 && !force) {
+        std::vector<int64_t> off(g.nb + 2, 0);
+        for (int l = 1; l <= g.nb; ++l) off[l + 1] = off[l] + (l - jl_first(l, ishift) + 1);
+        bool rows_ok = true;
+        for (size_t m = 0; m + 2 < tri.size(); m += 3)
+            if (tri[m + 1] < jl_first(tri[m], ishift) || tri[m + 1] > tri[m]) rows_ok = false;
+        GemmPlan h = make_gemm_plan(g.nb, 1, g.K, g.Kstride, g.precision, (int)off[g.nb + 1]);
+        h.rowmode = 1;
+        h.ish = ishift;
+        h.ncol_l = g.nb;
+        h.moff = 0;
+        h.mdiv = 1;
+        auto entry = [&](int l, int j, int i, int64_t &row, int64_t &c0, int64_t &c1) {
+            row = off[l] + (j - jl_first(l, ishift));
+            c0 = c1 = i - 1;
+        };
+        if (rows_ok && plan_windows(h, tri, entry, why)) {
+            h.win = true;
+            h.cld = g.nb;
+            h.cs_l = 1;
+            h.cs_e = 0;
+            h.coff = 0;
+            g = std::move(h);
+            return;
+        }
+    }
+    auto entry = [&](int l, int j, int i, int64_t &row, int64_t &c0, int64_t &c1) {
+        if (cross) {            // pair (j <= l) of field 1, field-2 column i
+            row = (int64_t)(l - 1) * l / 2 + (j - 1);
+            c0 = c1 = i - 1;
+        } else {                // pair (i <= j), columns (l, ell) for every ell
+            row = (int64_t)(j - 1) * j / 2 + (i - 1);
+            c0 = (int64_t)(l - 1) * g.nell;
+            c1 = c0 + g.nell - 1;
+        }
+    };
+    GemmPlan h = g;
+    if (!plan_windows(h, tri, entry, why)) {
+        PBI4_CHECK(!force, PBI4_EUNSUPPORTED, "PBI4_TRIANGLE_GEMM=block_sparse: " + why);
+        return;
+    }
+    // a work-group with every column tile gains nothing over the dense kernel's wider x-stages
+    // (64 x-values for up to 6 tiles) and single staging of each map
+    int64_t tiles = 0;
+    for (int b = 0; b < h.nrowblocks; ++b) tiles += h.wwidth[b];
+    if (!force && 5 * tiles > 4 * (int64_t)h.nrowblocks * h.cf) return;
+    h.win = true;
+    h.cld = h.ncol_l;
+    h.cs_l = cross ? 1 : h.nell;
+    h.cs_e = 1;
+    h.coff = 0;
+    g = std::move(h);
 }
 
 static GemmWindows windows_arg(const GemmPlan &g) {
@@ -833,7 +918,7 @@ static void launch_win(const GemmPlan &g, const float *maps, float *partials, hi
     const unsigned nwg = (unsigned)((int64_t)g.nrowblocks * g.nkchunks_pad);
     hipLaunchKernelGGL(kern, dim3(nwg), dim3(kGemmThreads), lds, s, maps, g.K, g.Kstride, g.nb,
                        g.moff, g.mdiv, g.ncol_l, g.npair, g.kchunk, g.nrowblocks, g.nkchunks,
-                       g.ucap, g.maxw * 16, windows_arg(g), partials);
+                       g.ucap, g.maxw * 16, g.rowmode, g.ish, windows_arg(g), partials);
 }
 
 template <int PREC, int CFW>
@@ -931,7 +1016,8 @@ void launch_bk_finalize(int rows, int64_t itot, const int32_t *tri, const double
                         double gridD, double sum_scale, double kcenter, double kbin, double *bout,
                         hipStream_t s) {
     hipLaunchKernelGGL(k_bk_finalize, dim3((unsigned)ceil_div(itot, 128)), dim3(128), 0, s, rows,
-                       itot, tri, C, g.cld, g.cs_l, g.cs_e, scale_exp, shell_sum, nmodes, counts,
+                       itot, tri, C, g.cld, g.cs_l, g.cs_e, g.rowmode, g.ish, scale_exp,
+                       shell_sum, nmodes, counts,
                        kf3, powSN, gridD, sum_scale, kcenter, kbin, bout);
     PBI4_HIP(hipGetLastError());
 }

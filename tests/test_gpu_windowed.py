@@ -113,3 +113,59 @@ def test_block_sparse_cross_equals_dense(ctx, case, prec):
     assert np.all(np.isfinite(b_w))
     bad = np.argwhere(b_w != b_d)
     assert bad.size == 0, f"{len(bad)} entries differ, first {bad[:5].tolist()}"
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("case", [(64, 8, 2.0, 2.0, True), (64, 20, 1.0, 1.0, True),
+                                  (64, 22, 1.0, 1.0, False), (96, 48, 1.0, 1.0, True),
+                                  (128, 64, 1.0, 1.0, True)],
+                         ids=lambda c: f"N{c[0]}_nb{c[1]}_kb{c[2]:g}_open{int(c[4])}")
+def test_real_space_rows_jl_match_dense(ctx, case, prec):
+    """Real space takes the rows (j, l) of the triangle loop with the leg i as the column (about
+    half the (i <= j) pair rows): the same triple sums paired and ordered differently, so some B
+    differ in the last bits from the (i <= j) block-sparse ones (bitwise the dense ones), which
+    shows the (j, l) form was taken.  Rows 0-5, 7, 8 (k's, powers, shot noise, N_tr) do not
+    involve the sums and are bitwise equal.  B is measured per bin against the dense kernel's
+    FP32 result, relative to max(|B|, sigma_B), sigma_B = sqrt(P1 P2 P3 / (N_tr kf^3)) the
+    estimator's Gaussian standard deviation: 1e-4 for both pairings in both precisions.  The
+    FP32 pairings differ by up to 6e-5 (64 shells on 128^3, where split-BF16 differs from FP32
+    by 3-4e-5 in either pairing): a different rounding of the pair products and of the FP32
+    chunk sums of strongly cancelling sums.  These small 64^3-particle fields put many bins at
+    noise level, where the split-BF16 rounding of either pairing reaches ~5e-3 of the stricter
+    floor sqrt(P1 P2 P3 / N_tr) of the FP64 oracle tests; on the real-space configurations those
+    tests hold both forms to 1e-4 of it."""
+    from pbi4_amd.catalogue import zeldovich
+    N, nb, kb, kc, iopen = case
+    print(f"real-space (j, l) rows vs dense: N={N} nb={nb} kb={kb} iopen={iopen} prec={prec}",
+          flush=True)
+    p, _ = zeldovich(N, box=1000.0, seed=11 + nb, device="cuda", velocities=False)
+    n = p.shape[1]
+    delta = ctx.density(ctx.assign(p, N, 4, True, box=1000.0, normalised=False), N, 4, True,
+                        1000.0, n)
+    itot = O.itot_count(nb, kb, kc, iopen)
+    ntr = ctx.count_triangles(nb, N, kb, kc, iopen, False, itot)
+    counts = _dev((ntr.astype(np.float64) * N ** 3).astype(np.float32))
+    kf = np.float32(2 * np.pi / 1000.0)
+
+    def desc(pr):
+        return ctx.bk_desc(nb, N, kb, kc, iopen, kf, itot, n, False, (0.0, 0.0, 1.0), pr, 0)
+
+    d = desc(prec)
+    _, b_jl = _run(ctx, None, lambda: ctx.bk(delta, counts, d), lambda: ctx.bk_workspace(d))
+    _, b_ij = _run(ctx, "block_sparse", lambda: ctx.bk(delta, counts, d),
+                   lambda: ctx.bk_workspace(d))
+    _, b_d = _run(ctx, "dense", lambda: ctx.bk(delta, counts, d), lambda: ctx.bk_workspace(d))
+    d32 = desc(0)
+    _, ref = _run(ctx, "dense", lambda: ctx.bk(delta, counts, d32), lambda: ctx.bk_workspace(d32))
+    assert not np.array_equal(b_jl[:, 6], b_ij[:, 6]), "the (j, l) row form was not taken"
+    assert np.array_equal(b_jl[:, [0, 1, 2, 3, 4, 5, 7, 8]], b_d[:, [0, 1, 2, 3, 4, 5, 7, 8]])
+    sigma_b = np.sqrt(np.abs(ref[:, 3] * ref[:, 4] * ref[:, 5])
+                      / (np.maximum(ref[:, 8], 1) * float(kf) ** 3))
+    bref = ref[:, 6] - ref[:, 7]                                 # B without the shot noise
+    scale = np.maximum(np.abs(bref), sigma_b)
+    e_jl = np.abs(b_jl[:, 6] - b_jl[:, 7] - bref) / scale
+    e_d = np.abs(b_d[:, 6] - b_d[:, 7] - bref) / scale
+    print(f"  max error vs dense FP32: (j, l) {e_jl.max():.3g}, (i, j) {e_d.max():.3g}", flush=True)
+    tol = 1e-4
+    assert e_jl.max() <= tol, f"max {e_jl.max():.3g} at triangle {int(e_jl.argmax())}"
+    assert e_d.max() <= tol
