@@ -205,10 +205,16 @@ __device__ __forceinline__ int hash_slot(uint32_t *keys, uint32_t t, uint32_t *n
     return -1;
 }
 
+// Direct counters (one per tile): the final flush adds the work-group's count of each tile it
+// touched to the tile's total and keeps the total before the add, this work-group's first slot
+// inside the tile, in wg_base[work-group][tile] (entries of untouched tiles are not written).
+// The scatter pass, over the same particles split the same way, starts its counters there and
+// ranks in LDS alone.  Hashed counters (more tiles than LDS holds) leave wg_base untouched.
 __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, int NT,
                                                             int64_t ntiles, const float *pos,
                                                             const float *vel, int64_t npart,
-                                                            uint32_t *counts, int hashed) {
+                                                            uint32_t *counts, uint32_t *wg_base,
+                                                            int hashed) {
     // hashed: values lhist[0..S), keys after them, then the number of keys in use; the table is
     // flushed to global memory and cleared whenever it is a quarter full, so a catalogue in any
     // order keeps its probes short
@@ -264,8 +270,27 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_hist(PosXform x, int T, i
         }
     }
     __syncthreads();
-    for (int64_t t = threadIdx.x; t < nctr; t += blockDim.x)
-        if (lhist[t]) atomicAdd(&counts[hashed ? keys[t] : t], lhist[t]);
+    if (hashed) {
+        for (int64_t t = threadIdx.x; t < nctr; t += blockDim.x)
+            if (lhist[t]) atomicAdd(&counts[keys[t]], lhist[t]);
+        return;
+    }
+    // four tiles per thread at a time: the four returning atomics are in flight together
+    uint32_t *row = wg_base + (int64_t)blockIdx.x * ntiles;
+    for (int64_t t0 = threadIdx.x; t0 < nctr; t0 += 4 * (int64_t)blockDim.x) {
+        uint32_t b[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t t = t0 + (int64_t)k * blockDim.x;
+            const uint32_t c = t < nctr ? lhist[t] : 0u;
+            b[k] = c ? atomicAdd(&counts[t], c) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t t = t0 + (int64_t)k * blockDim.x;
+            if (t < nctr && lhist[t]) row[t] = b[k];
+        }
+    }
 }
 
 // Exclusive scan of the tile counts by one work-group; offsets[ntiles] = npart; cursor = offsets.
@@ -295,6 +320,57 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const uint32_t *counts, int6
 
 constexpr int kScatterPer = 4;   // particles per thread per LDS round
 
+// Direct counters: one LDS counter per tile starting at the work-group's first slot in the tile
+// (offsets + wg_base from the histogram pass); a run's atomic add returns its final slots, so the
+// pass needs no global atomics and no barrier per round, whatever the particles' order.
+__global__ __launch_bounds__(kSortThreads) void k_tile_scatter_direct(PosXform x, int T, int NT,
+                                                                      int64_t ntiles,
+                                                                      const float *pos,
+                                                                      const float *vel,
+                                                                      int64_t npart,
+                                                                      const uint32_t *offsets,
+                                                                      const uint32_t *wg_base,
+                                                                      Pk3 *spk) {
+    extern __shared__ uint32_t lhist[];
+    const int64_t per_wg = ceil_div(npart, (int64_t)gridDim.x);
+    const int64_t beg = (int64_t)blockIdx.x * per_wg;
+    const int64_t end = min(npart, beg + per_wg);
+    // entries of tiles this work-group does not touch were never written and are never used
+    const uint32_t *row = wg_base + (int64_t)blockIdx.x * ntiles;
+    for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) lhist[t] = offsets[t] + row[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t round = (int64_t)blockDim.x * kScatterPer;
+    for (int64_t r0 = beg; r0 < end; r0 += round) {
+        RawParticle raw[kScatterPer];
+#pragma unroll
+        for (int k = 0; k < kScatterPer; ++k) {
+            const int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
+            raw[k] = load_particle(x, pos, vel, i < end ? i : beg, npart);
+        }
+#pragma unroll
+        for (int k = 0; k < kScatterPer; ++k) {
+            const int64_t i = r0 + threadIdx.x + (int64_t)k * blockDim.x;
+            uint32_t pk[3];
+            const int tl = i < end ? tile_of(x, T, NT, raw[k], pk) : -1;
+            int rr;
+            bool tail;
+            run_rank(tl, rr, tail);
+            uint32_t base = 0;
+            if (tl >= 0 && tail) base = atomicAdd(&lhist[tl], (uint32_t)(rr + 1));
+            // the tail hands the run's first slot to the head (non-tails push to lane + 1, never
+            // a head, so every head receives exactly one value); members read it from the head
+            const int dst = tail ? lane - rr : lane + 1;
+            const uint32_t b_head = (uint32_t)__builtin_amdgcn_ds_permute(4 * dst, (int)base);
+            const uint32_t slot = (uint32_t)__shfl((int)b_head, lane - rr) + (uint32_t)rr;
+            if (tl < 0 || slot >= (uint32_t)npart) continue;   // a bad slot cannot happen
+            spk[slot] = Pk3{{pk[0], pk[1], pk[2]}};
+        }
+    }
+}
+
+// Hashed counters (more tiles than LDS holds): per round, counters in a hash table, one global
+// atomic per touched tile on the cursors, then the writes.
 __global__ __launch_bounds__(kSortThreads) void k_tile_scatter(PosXform x, int T, int NT,
                                                                int64_t ntiles, const float *pos,
                                                                const float *vel, int64_t npart,
@@ -961,6 +1037,15 @@ AssignPlan make_assign_plan(const pbi4_assign_desc &d, int64_t npart) {
     return p;
 }
 
+// Work-groups of the two sort passes (both split the particles the same way).
+static int sort_wgs(int64_t npart) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(2048, ceil_div(npart, 8192)));
+}
+// Per-(work-group, tile) first slots of the direct-counter sort (none for hashed counters).
+static size_t sort_base_entries(const AssignPlan &p, int64_t npart) {
+    return p.ntiles > kLdsDirectMax ? 0 : (size_t)sort_wgs(npart) * (size_t)p.ntiles;
+}
+
 size_t assign_workspace(const AssignPlan &p, int64_t npart) {
     Arena a(nullptr, 0);
     a.take<uint32_t>(p.ntiles + 1);     // counts
@@ -968,6 +1053,7 @@ size_t assign_workspace(const AssignPlan &p, int64_t npart) {
     a.take<uint32_t>(p.ntiles + 1);     // cursor
     a.take<Pk3>(npart);                 // sorted packed coordinates
     a.take<float>((size_t)p.ntiles * p.ngrids * p.R * p.R * p.R);   // regions
+    a.take<uint32_t>(sort_base_entries(p, npart) + 1);              // wg_base
     return a.used + 256;
 }
 
@@ -982,6 +1068,7 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
     Pk3 *spk = ws.take<Pk3>(npart);
     const int R3 = p.R * p.R * p.R;
     float *regions = ws.take<float>((size_t)p.ntiles * p.ngrids * R3);
+    uint32_t *wg_base = ws.take<uint32_t>(sort_base_entries(p, npart) + 1);
 
     PosXform x{};
     x.N = p.N;
@@ -993,18 +1080,20 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
     const int hashed = p.ntiles > kLdsDirectMax;
     const size_t lds = (hashed ? 2 * (size_t)kHashSlots + 1 : (size_t)p.ntiles) * sizeof(uint32_t);
     const size_t lds_scatter = lds + ((size_t)kSortThreads * kScatterPer + 1) * sizeof(uint32_t);
-    const int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(2048, ceil_div(npart, 8192)));
-    const int nwg_h = nwg, nwg_s = nwg;
+    const int nwg = sort_wgs(npart);   // the same split in both passes (wg_base relies on it)
 
     {
         TimedScope ts(timer, "pbi4_sort", s);
         launch_fill_u32(counts, 0u, p.ntiles + 1, s);
         if (npart > 0)
-            hipLaunchKernelGGL(k_tile_hist, dim3(nwg_h), dim3(kSortThreads), lds, s, x, p.T, p.NT,
-                               p.ntiles, pos, vel, npart, counts, hashed);
+            hipLaunchKernelGGL(k_tile_hist, dim3(nwg), dim3(kSortThreads), lds, s, x, p.T, p.NT,
+                               p.ntiles, pos, vel, npart, counts, wg_base, hashed);
         hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, p.ntiles, offsets, cursor);
-        if (npart > 0)
-            hipLaunchKernelGGL(k_tile_scatter, dim3(nwg_s), dim3(kSortThreads), lds_scatter, s, x, p.T,
+        if (npart > 0 && !hashed)
+            hipLaunchKernelGGL(k_tile_scatter_direct, dim3(nwg), dim3(kSortThreads), lds, s, x,
+                               p.T, p.NT, p.ntiles, pos, vel, npart, offsets, wg_base, spk);
+        else if (npart > 0)
+            hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(kSortThreads), lds_scatter, s, x, p.T,
                                p.NT, p.ntiles, pos, vel, npart, cursor, spk, hashed);
     }
     {

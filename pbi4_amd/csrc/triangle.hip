@@ -710,7 +710,7 @@ __global__ __launch_bounds__(kCountThreads) void k_count_pairs(int N, int nb,
 
 static int gemm_cf(int ncol) {
     const int cf = (int)ceil_div(ncol, 16);
-    for (int c : {1, 2, 3, 4, 6, 8, 9, 12})
+    for (int c : {1, 2, 3, 4, 6, 8, 9, kMaxCF})
         if (c >= cf) return c;
     return -1;
 }
