@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -432,6 +433,26 @@ int maps_batch(int Np, int nmaps) {
     return (int)ceil_div((int64_t)nmaps, runs);
 }
 
+// Band-limited maps whose planes fit LDS take the one-pass 2D c2r (k_c2r2d) instead of rocFFT's
+// two passes; the filter then writes only the columns x' < C = min(R + 1, N'/2 + 1) that can be
+// nonzero.  PBI4_MAPS_C2R=rocfft keeps rocFFT's batched 2D c2r (comparisons and timing).
+bool maps_lds(int Np, int grid, const ShellInfo &sh, int64_t Kstride, C2r2dPlan *plan = nullptr) {
+    if (!maps_fused(Np, grid, sh)) return false;
+    const char *mode = std::getenv("PBI4_MAPS_C2R");
+    if (mode && std::strcmp(mode, "rocfft") == 0) return false;
+    C2r2dPlan p;
+    const int C = std::min(sh.R + 1, Np / 2 + 1);
+    if (!make_c2r2d_plan(Np, C, maps_ld(Np, grid, sh), Kstride, p)) return false;
+    if (plan) *plan = p;
+    return true;
+}
+
+// Columns the fused z filter writes: C on the LDS c2r path, else all (0).
+int maps_cols(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
+    C2r2dPlan p;
+    return maps_lds(Np, grid, sh, Kstride, &p) ? p.C : 0;
+}
+
 FftPlan &maps_plan(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
                    int64_t Kstride) {
     if (!maps_fused(Np, grid, sh)) return get_plan(ctx, 1, Np, maps_batch(Np, nmaps), Kstride);
@@ -443,6 +464,7 @@ FftPlan &maps_plan(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nma
 // batch with a plan of its own).
 size_t maps_work(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
                  int64_t Kstride) {
+    if (maps_lds(Np, grid, sh, Kstride)) return 0;
     size_t w = maps_plan(ctx, sh, Np, grid, nmaps, Kstride).work;
     if (!maps_fused(Np, grid, sh)) {
         const int b = maps_batch(Np, nmaps);
@@ -453,6 +475,12 @@ size_t maps_work(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps
 
 void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, int64_t Kstride,
               float2 *spectra, float *maps, void *work, hipStream_t s) {
+    C2r2dPlan lp;
+    if (maps_lds(Np, grid, sh, Kstride, &lp)) {
+        TimedScope ts(&ctx->timer, "pbi4_c2r", s);
+        launch_c2r2d(lp, spectra, get_twiddle(ctx, Np), maps, nmaps, s);
+        return;
+    }
     FftPlan &p = maps_plan(ctx, sh, Np, grid, nmaps, Kstride);
     TimedScope ts(&ctx->timer, "pbi4_c2r", s);
     if (!maps_fused(Np, grid, sh)) {
@@ -702,6 +730,7 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
     f.max_d2 = sh.max_d2;
     f.shell_d2 = sh.d2range_d;
     f.ld = maps_ld(L.Np, d->grid, sh);
+    f.cols = maps_cols(L.Np, d->grid, sh, L.Kstride);
     f.f2 = static_cast<const float2 *>(delta2);
     f.f4 = static_cast<const float2 *>(delta4);
     PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
@@ -1136,6 +1165,7 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
             fp.max_d2 = sh.max_d2;
             fp.shell_d2 = sh.d2range_d;
             fp.ld = maps_ld(L.Np, d->grid, sh);
+            fp.cols = maps_cols(L.Np, d->grid, sh, L.Kstride);
             TimedScope ts(&ctx->timer, "pbi4_filter", s);
             if (maps_fused(L.Np, d->grid, sh))
                 launch_filter_z(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,

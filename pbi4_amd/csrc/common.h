@@ -115,12 +115,29 @@ struct FilterPlan {
     const int32_t *shell_d2;  // per shell: smallest and largest d2 (the fused z filter)
     int ld;                   // the fused z filter's output row stride (>= N'/2 + 1)
     const float2 *f2, *f4;    // explicit delta_2, delta_4 fields, or null for L_l(mu) delta
+    int cols;                 // fused z filter: columns x' < cols written (0 = all N'/2 + 1)
 };
 void launch_filter(const FilterPlan &f, const float2 *delta, const double *shell_sum,
                    const double *nmodes, float2 *spectra, float *scale_exp, hipStream_t s);
 // Band-limited maps grid (3R < N' < N): shell filter fused with the inverse transform over z.
 void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *shell_sum,
                      const float2 *tw, float2 *spectra, float *scale_exp, hipStream_t s);
+
+// Inverse 2D transform over (y, x) of the band-limited maps' planes in one pass through LDS:
+// per plane, the half spectrum's columns x' < C (the rest is zero) go through inverse DFTs along
+// y, then each row's c2r as an inverse complex DFT of length N'/2 (the even / odd split of the
+// real output), written as real [y][x]; four-step transforms with compile-time factors for the
+// maps grids 100, 150 and 200.  Unnormalised, as a c2r of the same half spectrum.
+struct C2r2dPlan {
+    int Np, C, ld, yc;        // N', columns carried, input row stride, rows per LDS chunk
+    int64_t in_plane;         // complex entries per input plane (N' ld)
+    int64_t Kstride;          // floats per output map
+    size_t lds;               // dynamic LDS bytes
+};
+// False when N' has no compiled transform or the planes do not fit LDS.
+bool make_c2r2d_plan(int Np, int C, int ld, int64_t Kstride, C2r2dPlan &p);
+void launch_c2r2d(const C2r2dPlan &p, const float2 *spectra, const float2 *tw, float *maps,
+                  int nmaps, hipStream_t s);
 
 // ---------------------------------------------------------------- triangles (triangle.hip)
 struct GemmPlan {

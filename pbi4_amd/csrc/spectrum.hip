@@ -447,7 +447,7 @@ __device__ __forceinline__ int isqrt_floor(int v) {
 template <int ZC, int NELL>
 __global__ __launch_bounds__(256) void k_filter_z(FilterPlan f, const float2 *delta, const float *scale_exp,
                            const float2 *tw, float2 *out) {
-    const int Np = f.Np, hp = Np / 2 + 1;
+    const int Np = f.Np, hp = f.cols ? f.cols : Np / 2 + 1;   // columns x' < hp written
     const int line = blockIdx.x * blockDim.x + threadIdx.x;  // y * hp + x'
     if (line >= Np * hp) return;
     const int s = blockIdx.y + 1, z0 = blockIdx.z * ZC;
@@ -635,6 +635,198 @@ __global__ void k_qk(int N, int norder, int4 idx, const float2 *F, float2 *out) 
     out[t] = cscale(F[t], f);
 }
 
+
+// ------------------------------------------------------------------ maps: 2D c2r through LDS
+
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 times_i(float2 a) { return make_float2(-a.y, a.x); }
+
+// In-place inverse DFT of R values, y_k = sum_m a_m e^{+2 pi i m k / R}.  Constants are cos and sin
+// of 2 pi / 3, 2 pi / 5 and 4 pi / 5 rounded once from their double values.
+template <int R>
+__device__ __forceinline__ void idft(float2 *a) {
+    if constexpr (R == 2) {
+        const float2 t = a[0];
+        a[0] = cadd(t, a[1]);
+        a[1] = csub(t, a[1]);
+    } else if constexpr (R == 3) {
+        const float2 s = cadd(a[1], a[2]);
+        const float2 u = times_i(cscale(csub(a[1], a[2]), 0.8660254037844386f));
+        const float2 t = make_float2(a[0].x - 0.5f * s.x, a[0].y - 0.5f * s.y);
+        a[0] = cadd(a[0], s);
+        a[1] = cadd(t, u);
+        a[2] = csub(t, u);
+    } else if constexpr (R == 4) {
+        const float2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        const float2 s13 = cadd(a[1], a[3]), d13 = times_i(csub(a[1], a[3]));
+        a[0] = cadd(s02, s13);
+        a[2] = csub(s02, s13);
+        a[1] = cadd(d02, d13);
+        a[3] = csub(d02, d13);
+    } else {
+        const float c1 = 0.30901699437494745f, c2 = -0.8090169943749475f;
+        const float s1 = 0.9510565162951535f, s2 = 0.5877852522924731f;
+        const float2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+        const float2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+        const float2 t1 = make_float2(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
+        const float2 t2 = make_float2(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
+        const float2 u1 = times_i(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+        const float2 u2 = times_i(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+        a[0] = cadd(a[0], cadd(b1, b2));
+        a[1] = cadd(t1, u1);
+        a[4] = csub(t1, u1);
+        a[2] = cadd(t2, u2);
+        a[3] = csub(t2, u2);
+    }
+}
+
+// Compile-time inverse DFT of L values in registers, natural order in and out:
+// X[k0 + M k1] = sum_m W_L^{m k0} W_r^{m k1} Y_m[k0] with L = r M, Y_m the length-M transform of
+// x[m + r i] (decimation in time); radices 5, 4, 3, 2 first to last.  The twiddles W_L^e =
+// e^{2 pi i e / L} are compile-time constants from a Taylor series in double, rounded once.
+constexpr double kPiD = 3.14159265358979323846;
+constexpr double cx_sin(double x) {
+    double term = x, sum = x;
+    for (int n = 1; n < 30; ++n) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double cx_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 30; ++n) {
+        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+template <int L>
+struct TwTab {
+    float re[L], im[L];
+    constexpr TwTab() : re(), im() {
+        for (int e = 0; e < L; ++e) {
+            const int f = 2 * e > L ? e - L : e;            // angle in [-pi, pi]
+            const double a = 2.0 * kPiD * (double)f / (double)L;
+            re[e] = (float)cx_cos(a);
+            im[e] = (float)cx_sin(a);
+        }
+    }
+};
+template <int L>
+__device__ __forceinline__ float2 tw_const(int e) {
+    constexpr TwTab<L> t{};
+    return make_float2(t.re[e % L], t.im[e % L]);
+}
+
+template <int L>
+__device__ __forceinline__ void rdft(float2 *x) {
+    if constexpr (L == 1) {
+        return;
+    } else if constexpr (L == 2 || L == 3 || L == 4 || L == 5) {
+        idft<L>(x);
+    } else {
+        constexpr int r = L % 5 == 0 ? 5 : L % 4 == 0 ? 4 : L % 3 == 0 ? 3 : 2;
+        constexpr int M = L / r;
+        static_assert(r * M == L, "lengths are 2,3,5-smooth");
+        float2 sub[r][M];
+#pragma unroll
+        for (int m = 0; m < r; ++m) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) sub[m][i] = x[m + r * i];
+            rdft<M>(sub[m]);
+        }
+#pragma unroll
+        for (int k0 = 0; k0 < M; ++k0) {
+            float2 a[r];
+#pragma unroll
+            for (int m = 0; m < r; ++m) a[m] = m * k0 ? cmul(sub[m][k0], tw_const<L>(m * k0)) : sub[m][k0];
+            idft<r>(a);
+#pragma unroll
+            for (int k1 = 0; k1 < r; ++k1) x[k0 + M * k1] = a[k1];
+        }
+    }
+}
+
+// One work-group per plane (map m, z); four-step transforms with compile-time factors.
+// Columns (length N' = A1 B1, only x' < C): thread (c, j < B1) loads rows q = j + B1 m, m < A1,
+// transforms them over m, multiplies by W_N'^{j k1} and stores at k1 B1 + j; thread (c, k1 < A1)
+// then transforms positions k1 B1 + j over j in place, so row y's value sits at
+// (y mod A1) B1 + y / A1 (W_N'^e = twl[e], W_{N'/2}^e = twl[2e]).  Rows (the c2r of row y as an inverse DFT of length M = N'/2 = A2 B2):
+// with X[k] the column outputs (zero for k >= C) and W = e^{2 pi i / N'},
+// Z[k] = (X[k] + X*[M-k]) + i W^k (X[k] - X*[M-k]); the same two steps over Z give z, and
+// x[2n] = Re z[n], x[2n+1] = Im z[n] are the real outputs.  LDS: twiddles W^k, the columns (stride
+// N' + 1 entries) and a chunk of yc rows.
+template <int NP, int A1, int B1, int A2, int B2>
+__global__ __launch_bounds__(256) void k_c2r2d(C2r2dPlan p, const float2 *spec, const float2 *tw,
+                                               float *maps) {
+    static_assert(A1 * B1 == NP && 2 * A2 * B2 == NP, "factors of N' and N'/2");
+    constexpr int M = NP / 2, CS = NP + 1;
+    extern __shared__ float2 sm[];
+    const int C = p.C;
+    float2 *twl = sm, *col = sm + NP, *row = col + C * CS;
+    const int plane = blockIdx.x;
+    const int map = plane / NP, z = plane - map * NP;
+    for (int i = threadIdx.x; i < NP; i += blockDim.x) twl[i] = tw[i];
+    const float2 *in = spec + (int64_t)plane * p.in_plane;
+    for (int t = threadIdx.x; t < C * B1; t += blockDim.x) {
+        const int j = t / C, c = t - j * C;
+        float2 x[A1];
+#pragma unroll
+        for (int m = 0; m < A1; ++m) x[m] = in[(int64_t)(j + B1 * m) * p.ld + c];
+        rdft<A1>(x);
+        float2 *cc = col + c * CS + j;
+#pragma unroll
+        for (int k1 = 0; k1 < A1; ++k1) cc[k1 * B1] = k1 ? cmul(x[k1], twl[j * k1]) : x[0];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < C * A1; t += blockDim.x) {
+        const int k1 = t / C, c = t - k1 * C;
+        float2 *cc = col + c * CS + k1 * B1;
+        float2 x[B1];
+#pragma unroll
+        for (int j = 0; j < B1; ++j) x[j] = cc[j];
+        rdft<B1>(x);
+#pragma unroll
+        for (int j = 0; j < B1; ++j) cc[j] = x[j];
+    }
+    __syncthreads();
+    float *out = maps + (int64_t)map * p.Kstride + (int64_t)z * NP * NP;
+    for (int y0 = 0; y0 < NP; y0 += p.yc) {
+        const int ny = min(p.yc, NP - y0);
+        for (int t = threadIdx.x; t < ny * B2; t += blockDim.x) {
+            const int yl = t / B2, j = t - yl * B2, y = y0 + yl;
+            const int py = (y % A1) * B1 + y / A1;
+            float2 x[A2];
+#pragma unroll
+            for (int m = 0; m < A2; ++m) {
+                const int k = j + B2 * m;
+                float2 xk = make_float2(0.0f, 0.0f), xm = xk;
+                if (k < C) xk = col[k * CS + py];
+                if (M - k < C) xm = conjf2(col[(M - k) * CS + py]);
+                x[m] = cadd(cadd(xk, xm), times_i(cmul(csub(xk, xm), twl[k])));
+            }
+            rdft<A2>(x);
+            float2 *rr = row + yl * M + j;
+#pragma unroll
+            for (int k1 = 0; k1 < A2; ++k1) rr[k1 * B2] = k1 ? cmul(x[k1], twl[2 * j * k1]) : x[0];
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < ny * A2; t += blockDim.x) {
+            const int yl = t / A2, k1 = t - yl * A2;
+            const float2 *rr = row + yl * M + k1 * B2;
+            float2 x[B2];
+#pragma unroll
+            for (int j = 0; j < B2; ++j) x[j] = rr[j];
+            rdft<B2>(x);
+            float2 *o = reinterpret_cast<float2 *>(out + (int64_t)(y0 + yl) * NP);
+#pragma unroll
+            for (int k2 = 0; k2 < B2; ++k2) o[k1 + A2 * k2] = x[k2];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
@@ -772,6 +964,49 @@ void launch_bin_finalize(const BinPlan &b, const float *partials, double kF3, do
 }
 
 
+// The maps grids with a compiled plane transform: N' = 100 (the 256^3 workload's 16 shells of
+// 2 k_F), 150 (48 shells of k_F at 512^3), 200 (32 shells of 2 k_F at 1024^3).
+bool make_c2r2d_plan(int Np, int C, int ld, int64_t Kstride, C2r2dPlan &p) {
+    if (Np != 100 && Np != 150 && Np != 200) return false;
+    if (C < 1 || C > Np / 2 + 1 || ld < C) return false;
+    p = C2r2dPlan{};
+    p.Np = Np;
+    p.C = C;
+    p.ld = ld;
+    p.in_plane = (int64_t)Np * ld;
+    p.Kstride = Kstride;
+    // rows per chunk: all of them within 64 KB if possible (several work-groups per CU), else
+    // within 160 KB, in chunks of equal size
+    const size_t fixed = sizeof(float2) * ((size_t)Np + (size_t)C * (Np + 1));
+    const size_t per_row = sizeof(float2) * (size_t)(Np / 2);
+    for (size_t budget : {(size_t)64 << 10, (size_t)160 << 10}) {
+        if (fixed + 8 * per_row > budget) continue;
+        const int most = (int)std::min<size_t>(Np, (budget - fixed) / per_row);
+        const int chunks = (Np + most - 1) / most;
+        p.yc = (Np + chunks - 1) / chunks;
+        p.lds = fixed + per_row * p.yc;
+        return true;
+    }
+    return false;
+}
+
+void launch_c2r2d(const C2r2dPlan &p, const float2 *spectra, const float2 *tw, float *maps,
+                  int nmaps, hipStream_t s) {
+    auto launch = [&](auto kern) {
+        if (p.lds > 64 * 1024)
+            PBI4_HIP(hipFuncSetAttribute((const void *)kern,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds));
+        hipLaunchKernelGGL(kern, dim3((unsigned)((int64_t)nmaps * p.Np)), dim3(256), p.lds, s, p,
+                           spectra, tw, maps);
+    };
+    switch (p.Np) {
+        case 100: launch(k_c2r2d<100, 10, 10, 10, 5>); break;
+        case 150: launch(k_c2r2d<150, 10, 15, 15, 5>); break;
+        default: launch(k_c2r2d<200, 10, 20, 10, 10>); break;
+    }
+    PBI4_HIP(hipGetLastError());
+}
+
 void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *shell_sum,
                      const float2 *tw, float2 *spectra, float *scale_exp, hipStream_t s) {
     PBI4_CHECK(3 * f.R < f.Np && f.Np < f.N, PBI4_EINVAL,
@@ -781,7 +1016,8 @@ void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *she
     PBI4_HIP(hipGetLastError());
     PBI4_CHECK(f.shell_d2 != nullptr, PBI4_EINVAL, "the fused z filter needs the shell d2 ranges");
     PBI4_CHECK(f.ld > f.Np / 2, PBI4_EINVAL, "the fused z filter's row stride is too small");
-    const int lines = f.Np * (f.Np / 2 + 1);
+    PBI4_CHECK(f.cols >= 0 && f.cols <= f.Np / 2 + 1, PBI4_EINVAL, "bad fused z filter columns");
+    const int lines = f.Np * (f.cols ? f.cols : f.Np / 2 + 1);
     // planes per thread: 20 with three multipole maps (120 accumulator registers), else 25
     if (f.nell == 3)
         hipLaunchKernelGGL((k_filter_z<20, 3>),

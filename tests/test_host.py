@@ -201,12 +201,18 @@ def test_assign_tile_edge_follows_particle_density():
     def regions(N, T):
         return (N // T) ** 3 * 2 * (T + 4) ** 3 * 4
 
+    def bases(N, T, npart):
+        # with at most 8192 tiles the sort keeps each work-group's first slot per tile: one word
+        # per (work-group, tile), min(2048, ceil(npart / 8192)) work-groups
+        ntiles = (N // T) ** 3
+        return 4 * max(1, min(2048, -(-npart // 8192))) * ntiles if ntiles <= 8192 else 0
+
     def rest(N, npart):                 # sort arrays and tile counters, below the regions
         return ws(N, npart) - 12 * npart
 
     for N, npart, T in [(128, 1_000_000, 16), (128, 1 << 24, 8), (256, 1 << 24, 16),
                         (192, 1 << 24, 12), (92, 60000, 4), (20, 1 << 20, 10)]:
-        r = rest(N, npart)
+        r = rest(N, npart) - bases(N, T, npart)
         assert regions(N, T) <= r <= regions(N, T) + 3 * 4 * ((N // T) ** 3 + 1) + 8192, (N, npart)
 
 
