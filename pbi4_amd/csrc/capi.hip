@@ -408,17 +408,59 @@ float2 *get_twiddle(pbi4_ctx *ctx, int Np) {
     return ctx->twiddles[Np] = dev_upload(ctx, t);
 }
 
-// The maps' inverse transform.  Band-limited maps grid (N' < N, so 3R < N'): the z transform is
-// fused into the shell filter (k_filter_z) and a batched 2D c2r over (y, x) finishes; one plan
-// over all maps when their stride is N'^3, else one execution per map.  Full grid: the shell
-// filter writes half spectra and a 3D c2r transforms them.
-bool maps_fused(int Np, int grid, const ShellInfo &sh) { return Np < grid && sh.contiguous; }
+// The maps' inverse transform, one of four routes:
+//  * band-limited maps grid (N' < N, so 3R < N'): the z transform is fused into the shell filter
+//    (k_filter_z); the 2D c2r over (y, x) that finishes each plane is k_c2r2d (one pass through
+//    LDS) on the maps grids it is compiled for, else rocFFT's batched 2D c2r;
+//  * full grid (N' = N, an aliasing request) with 2R < N on 128, 192, 256 or 384: the same fused
+//    filter, then the column and row passes of launch_maps_two_pass;
+//  * otherwise the shell filter writes half spectra and rocFFT's 3D c2r transforms them.
+// The fused routes write only the columns x' < C = min(R + 1, N'/2 + 1) that can be nonzero when
+// their transform reads no others.  PBI4_MAPS_C2R=rocfft keeps the rocFFT routes (comparisons
+// and timing).
+enum class MapsPath { Rocfft3d, Rocfft2d, LdsPlane, TwoPass };
+struct MapsRoute {
+    MapsPath path = MapsPath::Rocfft3d;
+    int ld = 0;          // row stride of the half spectra (complex entries)
+    int cols = 0;        // columns the fused filter writes (0 = all N'/2 + 1)
+    C2r2dPlan plane{};   // LdsPlane
+};
 
-// Row stride of the maps' half spectra: N'/2 + 1, padded to a multiple of 64 entries on the
-// fused path (512-byte rows for the 2D c2r's column pass).
-int maps_ld(int Np, int grid, const ShellInfo &sh) {
-    const int h = Np / 2 + 1;
-    return maps_fused(Np, grid, sh) ? (h + 63) / 64 * 64 : h;
+MapsRoute maps_route(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
+    MapsRoute r;
+    const int h = Np / 2 + 1, C = std::min(sh.R + 1, h);
+    const char *mode = std::getenv("PBI4_MAPS_C2R");
+    const bool ours = !(mode && std::strcmp(mode, "rocfft") == 0);
+    if (Np < grid && sh.contiguous) {
+        // rows padded to 64 entries: 512-byte rows for rocFFT's column pass
+        r.ld = (h + 63) / 64 * 64;
+        if (ours && make_c2r2d_plan(Np, C, r.ld, Kstride, r.plane)) {
+            r.path = MapsPath::LdsPlane;
+            r.cols = C;
+        } else {
+            r.path = MapsPath::Rocfft2d;
+        }
+        return r;
+    }
+    if (ours && Np == grid && sh.contiguous && 2 * sh.R < Np && maps_two_pass_supported(Np)) {
+        r.path = MapsPath::TwoPass;
+        r.cols = C;
+        r.ld = (C + 15) / 16 * 16;   // whole 16-column blocks for the column pass
+        return r;
+    }
+    r.ld = h;
+    return r;
+}
+
+// The shell filter with the z transform fused (every route but rocFFT's 3D c2r).
+bool maps_fused(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
+    return maps_route(Np, grid, sh, Kstride).path != MapsPath::Rocfft3d;
+}
+int maps_ld(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
+    return maps_route(Np, grid, sh, Kstride).ld;
+}
+int maps_cols(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
+    return maps_route(Np, grid, sh, Kstride).cols;
 }
 
 // Full-grid maps: one 3D c2r execution transforms at most this many real outputs (all its
@@ -433,40 +475,20 @@ int maps_batch(int Np, int nmaps) {
     return (int)ceil_div((int64_t)nmaps, runs);
 }
 
-// Band-limited maps whose planes fit LDS take the one-pass 2D c2r (k_c2r2d) instead of rocFFT's
-// two passes; the filter then writes only the columns x' < C = min(R + 1, N'/2 + 1) that can be
-// nonzero.  PBI4_MAPS_C2R=rocfft keeps rocFFT's batched 2D c2r (comparisons and timing).
-bool maps_lds(int Np, int grid, const ShellInfo &sh, int64_t Kstride, C2r2dPlan *plan = nullptr) {
-    if (!maps_fused(Np, grid, sh)) return false;
-    const char *mode = std::getenv("PBI4_MAPS_C2R");
-    if (mode && std::strcmp(mode, "rocfft") == 0) return false;
-    C2r2dPlan p;
-    const int C = std::min(sh.R + 1, Np / 2 + 1);
-    if (!make_c2r2d_plan(Np, C, maps_ld(Np, grid, sh), Kstride, p)) return false;
-    if (plan) *plan = p;
-    return true;
-}
-
-// Columns the fused z filter writes: C on the LDS c2r path, else all (0).
-int maps_cols(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
-    C2r2dPlan p;
-    return maps_lds(Np, grid, sh, Kstride, &p) ? p.C : 0;
-}
-
-FftPlan &maps_plan(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
-                   int64_t Kstride) {
-    if (!maps_fused(Np, grid, sh)) return get_plan(ctx, 1, Np, maps_batch(Np, nmaps), Kstride);
+FftPlan &maps_plan(pbi4_ctx *ctx, const MapsRoute &r, int Np, int nmaps, int64_t Kstride) {
+    if (r.path == MapsPath::Rocfft3d) return get_plan(ctx, 1, Np, maps_batch(Np, nmaps), Kstride);
     const bool one = Kstride == (int64_t)Np * Np * Np;
-    return get_plan(ctx, 3, Np, one ? nmaps * Np : Np, maps_ld(Np, grid, sh));
+    return get_plan(ctx, 3, Np, one ? nmaps * Np : Np, r.ld);
 }
 
 // Work buffer bytes the maps' c2r executions need (the full-grid path may run a shorter last
-// batch with a plan of its own).
+// batch with a plan of its own); none on the routes without rocFFT.
 size_t maps_work(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps,
                  int64_t Kstride) {
-    if (maps_lds(Np, grid, sh, Kstride)) return 0;
-    size_t w = maps_plan(ctx, sh, Np, grid, nmaps, Kstride).work;
-    if (!maps_fused(Np, grid, sh)) {
+    const MapsRoute r = maps_route(Np, grid, sh, Kstride);
+    if (r.path == MapsPath::LdsPlane || r.path == MapsPath::TwoPass) return 0;
+    size_t w = maps_plan(ctx, r, Np, nmaps, Kstride).work;
+    if (r.path == MapsPath::Rocfft3d) {
         const int b = maps_batch(Np, nmaps);
         if (nmaps % b) w = std::max(w, get_plan(ctx, 1, Np, nmaps % b, Kstride).work);
     }
@@ -475,15 +497,21 @@ size_t maps_work(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps
 
 void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, int64_t Kstride,
               float2 *spectra, float *maps, void *work, hipStream_t s) {
-    C2r2dPlan lp;
-    if (maps_lds(Np, grid, sh, Kstride, &lp)) {
+    const MapsRoute r = maps_route(Np, grid, sh, Kstride);
+    if (r.path == MapsPath::LdsPlane) {
         TimedScope ts(&ctx->timer, "pbi4_c2r", s);
-        launch_c2r2d(lp, spectra, get_twiddle(ctx, Np), maps, nmaps, s);
+        launch_c2r2d(r.plane, spectra, get_twiddle(ctx, Np), maps, nmaps, s);
         return;
     }
-    FftPlan &p = maps_plan(ctx, sh, Np, grid, nmaps, Kstride);
+    if (r.path == MapsPath::TwoPass) {
+        TimedScope ts(&ctx->timer, "pbi4_c2r", s);
+        launch_maps_two_pass(Np, r.cols, r.ld, nmaps, Kstride, get_twiddle(ctx, Np), spectra, maps,
+                             s);
+        return;
+    }
+    FftPlan &p = maps_plan(ctx, r, Np, nmaps, Kstride);
     TimedScope ts(&ctx->timer, "pbi4_c2r", s);
-    if (!maps_fused(Np, grid, sh)) {
+    if (r.path == MapsPath::Rocfft3d) {
         const int b = maps_batch(Np, nmaps);
         const size_t spec = (size_t)Np * Np * (Np / 2 + 1);
         for (int m0 = 0; m0 < nmaps; m0 += b) {
@@ -496,7 +524,7 @@ void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, i
         fft_exec(p, spectra, maps, work, s);
         return;
     }
-    const size_t spec = (size_t)Np * Np * maps_ld(Np, grid, sh);
+    const size_t spec = (size_t)Np * Np * r.ld;
     for (int m = 0; m < nmaps; ++m) fft_exec(p, spectra + m * spec, maps + m * Kstride, work, s);
 }
 
@@ -707,7 +735,7 @@ BkLayout bk_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a, int nbins_pk 
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(d.nbins);
     L.scale_exp = a.take<float>(d.nbins);
-    L.spectra = a.take<float2>((size_t)L.nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh));
+    L.spectra = a.take<float2>((size_t)L.nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh, L.Kstride));
     L.maps = a.take<float>((size_t)L.nmaps * L.Kstride);
     L.fftwork = a.take<char>(std::max<size_t>(work, 1));
     L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
@@ -729,7 +757,7 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
     f.shell_table = sh.table_d;
     f.max_d2 = sh.max_d2;
     f.shell_d2 = sh.d2range_d;
-    f.ld = maps_ld(L.Np, d->grid, sh);
+    f.ld = maps_ld(L.Np, d->grid, sh, L.Kstride);
     f.cols = maps_cols(L.Np, d->grid, sh, L.Kstride);
     f.f2 = static_cast<const float2 *>(delta2);
     f.f4 = static_cast<const float2 *>(delta4);
@@ -737,7 +765,7 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
                "delta2/delta4 are only used with multipoles");
     {
         TimedScope ts(&ctx->timer, "pbi4_filter", s);
-        if (maps_fused(L.Np, d->grid, sh))
+        if (maps_fused(L.Np, d->grid, sh, L.Kstride))
             launch_filter_z(f, static_cast<const float2 *>(delta), L.shell_sum,
                             get_twiddle(ctx, L.Np), L.spectra, L.scale_exp, s);
         else
@@ -795,7 +823,7 @@ BkCrossLayout bk_cross_layout(pbi4_ctx *ctx, const pbi4_bk_desc &d, Arena &a) {
     L.bin_partials = a.take<float>(bin_partials_floats(b));
     L.shell_sum = a.take<double>(nmaps);
     L.scale_exp = a.take<float>(nmaps);
-    L.spectra = a.take<float2>((size_t)nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh));
+    L.spectra = a.take<float2>((size_t)nmaps * L.Np * L.Np * maps_ld(L.Np, d.grid, sh, L.Kstride));
     L.maps = a.take<float>((size_t)nmaps * L.Kstride);
     L.fftwork = a.take<char>(std::max<size_t>(work, 1));
     L.gemm_partials = a.take<float>(gemm_partials_floats(L.g));
@@ -1145,7 +1173,7 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
         Arena a(ws, ws_bytes);
         BkCrossLayout L = bk_cross_layout(ctx, *d, a);
         const int nb = d->nbins;
-        const size_t spec = (size_t)L.Np * L.Np * maps_ld(L.Np, d->grid, sh);
+        const size_t spec = (size_t)L.Np * L.Np * maps_ld(L.Np, d->grid, sh, L.Kstride);
         const void *fields[2] = {delta1, delta2};
         for (int f = 0; f < 2; ++f) {
             // shell powers (Parseval) -> map scales; shell filter into maps f*nb .. f*nb+nb-1
@@ -1164,10 +1192,10 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
             fp.shell_table = sh.table_d;
             fp.max_d2 = sh.max_d2;
             fp.shell_d2 = sh.d2range_d;
-            fp.ld = maps_ld(L.Np, d->grid, sh);
+            fp.ld = maps_ld(L.Np, d->grid, sh, L.Kstride);
             fp.cols = maps_cols(L.Np, d->grid, sh, L.Kstride);
             TimedScope ts(&ctx->timer, "pbi4_filter", s);
-            if (maps_fused(L.Np, d->grid, sh))
+            if (maps_fused(L.Np, d->grid, sh, L.Kstride))
                 launch_filter_z(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,
                                 get_twiddle(ctx, L.Np), L.spectra + f * nb * spec,
                                 L.scale_exp + f * nb, s);

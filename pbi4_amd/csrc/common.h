@@ -138,6 +138,11 @@ struct C2r2dPlan {
 bool make_c2r2d_plan(int Np, int C, int ld, int64_t Kstride, C2r2dPlan &p);
 void launch_c2r2d(const C2r2dPlan &p, const float2 *spectra, const float2 *tw, float *maps,
                   int nmaps, hipStream_t s);
+// Full-grid maps (N' = N in 128, 192, 256, 384): after the fused z filter, the inverse DFTs along
+// y of the columns x' < C in place (row stride ld), then each row's c2r into the maps.
+bool maps_two_pass_supported(int Np);
+void launch_maps_two_pass(int Np, int C, int ld, int nmaps, int64_t Kstride, const float2 *tw,
+                          float2 *spectra, float *maps, hipStream_t s);
 
 // ---------------------------------------------------------------- triangles (triangle.hip)
 struct GemmPlan {

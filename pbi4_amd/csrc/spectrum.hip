@@ -827,6 +827,102 @@ __global__ __launch_bounds__(256) void k_c2r2d(C2r2dPlan p, const float2 *spec, 
     }
 }
 
+
+// Full-grid maps (N' = N, aliasing requests), two passes through HBM for planes too large for LDS.
+// Column pass: work-group (plane p = blockIdx.x, block blockIdx.y of kColBlock columns x');
+// thread (c, j < B) loads rows q = j + B m, m < A, of its column, transforms them over m,
+// multiplies by W_N^{j k1} and stores them to LDS at k1 B + j; thread (c, k1 < A) transforms
+// positions k1 B + j over j and writes X[k1 + A k2] back to row k1 + A k2 of the same column (in
+// place: a work-group owns whole columns).
+constexpr int kColBlock = 16;
+template <int NP, int A, int B>
+__global__ __launch_bounds__(256) void k_maps_cols(int C, int ld, const float2 *tw, float2 *spec) {
+    static_assert(A * B == NP, "factors of N");
+    constexpr int CS = NP + 1;
+    __shared__ float2 twl[NP];
+    __shared__ float2 col[kColBlock * CS];
+    const int cb0 = blockIdx.y * kColBlock;
+    float2 *pl = spec + (int64_t)blockIdx.x * NP * ld;
+    for (int i = threadIdx.x; i < NP; i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
+    for (int t = threadIdx.x; t < kColBlock * B; t += blockDim.x) {
+        const int j = t / kColBlock, c = t - j * kColBlock;
+        if (cb0 + c >= C) continue;
+        float2 x[A];
+#pragma unroll
+        for (int m = 0; m < A; ++m) x[m] = pl[(int64_t)(j + B * m) * ld + cb0 + c];
+        rdft<A>(x);
+        float2 *cc = col + c * CS + j;
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) cc[k1 * B] = k1 ? cmul(x[k1], twl[j * k1]) : x[0];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kColBlock * A; t += blockDim.x) {
+        const int k1 = t / kColBlock, c = t - k1 * kColBlock;
+        if (cb0 + c >= C) continue;
+        const float2 *cc = col + c * CS + k1 * B;
+        float2 x[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) x[j] = cc[j];
+        rdft<B>(x);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) pl[(int64_t)(k1 + A * k2) * ld + cb0 + c] = x[k2];
+    }
+}
+
+// Row pass: work-group of kRowBlock rows (plane p = map N + z, row y); the c2r of a row as one
+// inverse DFT of length M = N/2 = A2 B2 of Z[k] = (X[k] + X*[M-k]) + i W^k (X[k] - X*[M-k])
+// (X[k] = 0 for k >= C), in the same two steps; x[2n] = Re z[n], x[2n+1] = Im z[n] are written to
+// map p / N at z N^2 + y N.
+constexpr int kRowBlock = 8;
+template <int NP, int A2, int B2>
+__global__ __launch_bounds__(256) void k_maps_rows(int C, int ld, int64_t nrows, int64_t Kstride,
+                                                   const float2 *tw, const float2 *spec,
+                                                   float *maps) {
+    static_assert(2 * A2 * B2 == NP, "factors of N/2");
+    constexpr int M = NP / 2;
+    __shared__ float2 twl[NP];
+    __shared__ float2 row[kRowBlock * M];
+    const int64_t r0 = (int64_t)blockIdx.x * kRowBlock;
+    for (int i = threadIdx.x; i < NP; i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
+    for (int t = threadIdx.x; t < kRowBlock * B2; t += blockDim.x) {
+        const int r = t / B2, j = t - r * B2;
+        if (r0 + r >= nrows) continue;
+        const float2 *X = spec + (r0 + r) * ld;     // rows of all planes are ld apart
+        float2 x[A2];
+#pragma unroll
+        for (int m = 0; m < A2; ++m) {
+            const int k = j + B2 * m;
+            float2 xk = make_float2(0.0f, 0.0f), xm = xk;
+            if (k < C) xk = X[k];
+            if (M - k < C) xm = conjf2(X[M - k]);
+            x[m] = cadd(cadd(xk, xm), times_i(cmul(csub(xk, xm), twl[k])));
+        }
+        rdft<A2>(x);
+        float2 *rr = row + r * M + j;
+#pragma unroll
+        for (int k1 = 0; k1 < A2; ++k1) rr[k1 * B2] = k1 ? cmul(x[k1], twl[2 * j * k1]) : x[0];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kRowBlock * A2; t += blockDim.x) {
+        const int r = t / A2, k1 = t - r * A2;
+        const int64_t g = r0 + r;
+        if (g >= nrows) continue;
+        const float2 *rr = row + r * M + k1 * B2;
+        float2 x[B2];
+#pragma unroll
+        for (int j = 0; j < B2; ++j) x[j] = rr[j];
+        rdft<B2>(x);
+        const int64_t p = g / NP;                   // plane = map N + z
+        const int y = (int)(g - p * NP);
+        const int64_t map = p / NP, z = p - map * NP;
+        float2 *o = reinterpret_cast<float2 *>(maps + map * Kstride + (z * NP + y) * NP);
+#pragma unroll
+        for (int k2 = 0; k2 < B2; ++k2) o[k1 + A2 * k2] = x[k2];
+    }
+}
+
 }  // namespace
 
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
@@ -1007,16 +1103,51 @@ void launch_c2r2d(const C2r2dPlan &p, const float2 *spectra, const float2 *tw, f
     PBI4_HIP(hipGetLastError());
 }
 
+bool maps_two_pass_supported(int Np) { return Np == 128 || Np == 192 || Np == 256 || Np == 384; }
+
+void launch_maps_two_pass(int Np, int C, int ld, int nmaps, int64_t Kstride, const float2 *tw,
+                          float2 *spectra, float *maps, hipStream_t s) {
+    PBI4_CHECK(maps_two_pass_supported(Np) && C >= 1 && C <= Np / 2 && ld >= C, PBI4_EINVAL,
+               "full-grid maps transform: unsupported size");
+    const dim3 gc((unsigned)((int64_t)nmaps * Np), (unsigned)ceil_div(C, kColBlock));
+    const int64_t nrows = (int64_t)nmaps * Np * Np;
+    const dim3 gr((unsigned)ceil_div(nrows, kRowBlock));
+    switch (Np) {
+        case 128:
+            hipLaunchKernelGGL((k_maps_cols<128, 8, 16>), gc, dim3(256), 0, s, C, ld, tw, spectra);
+            hipLaunchKernelGGL((k_maps_rows<128, 8, 8>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
+                               tw, spectra, maps);
+            break;
+        case 192:
+            hipLaunchKernelGGL((k_maps_cols<192, 12, 16>), gc, dim3(256), 0, s, C, ld, tw, spectra);
+            hipLaunchKernelGGL((k_maps_rows<192, 8, 12>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
+                               tw, spectra, maps);
+            break;
+        case 256:
+            hipLaunchKernelGGL((k_maps_cols<256, 16, 16>), gc, dim3(256), 0, s, C, ld, tw, spectra);
+            hipLaunchKernelGGL((k_maps_rows<256, 8, 16>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
+                               tw, spectra, maps);
+            break;
+        default:
+            hipLaunchKernelGGL((k_maps_cols<384, 16, 24>), gc, dim3(256), 0, s, C, ld, tw, spectra);
+            hipLaunchKernelGGL((k_maps_rows<384, 12, 16>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
+                               tw, spectra, maps);
+            break;
+    }
+    PBI4_HIP(hipGetLastError());
+}
+
 void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *shell_sum,
                      const float2 *tw, float2 *spectra, float *scale_exp, hipStream_t s) {
-    PBI4_CHECK(3 * f.R < f.Np && f.Np < f.N, PBI4_EINVAL,
-               "the fused z filter needs a band-limited maps grid");
+    PBI4_CHECK(f.Np < f.N ? 3 * f.R < f.Np : (f.Np == f.N && 2 * f.R < f.Np), PBI4_EINVAL,
+               "the fused z filter needs a band-limited maps grid, or the full grid with 2R < N");
     hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)ceil_div(f.nshell, 64)), dim3(64), 0, s,
                        f.nshell, shell_sum, scale_exp);
     PBI4_HIP(hipGetLastError());
     PBI4_CHECK(f.shell_d2 != nullptr, PBI4_EINVAL, "the fused z filter needs the shell d2 ranges");
-    PBI4_CHECK(f.ld > f.Np / 2, PBI4_EINVAL, "the fused z filter's row stride is too small");
     PBI4_CHECK(f.cols >= 0 && f.cols <= f.Np / 2 + 1, PBI4_EINVAL, "bad fused z filter columns");
+    PBI4_CHECK(f.ld >= (f.cols ? f.cols : f.Np / 2 + 1), PBI4_EINVAL,
+               "the fused z filter's row stride is too small");
     const int lines = f.Np * (f.cols ? f.cols : f.Np / 2 + 1);
     // planes per thread: 20 with three multipole maps (120 accumulator registers), else 25
     if (f.nell == 3)
