@@ -415,14 +415,16 @@ float2 *get_twiddle(pbi4_ctx *ctx, int Np) {
 //  * full grid (N' = N, an aliasing request) with 2R < N on 128, 192, 256 or 384: the same fused
 //    filter, then the column and row passes of launch_maps_two_pass;
 //  * otherwise the shell filter writes half spectra and rocFFT's 3D c2r transforms them.
-// The fused routes write only the columns x' < C = min(R + 1, N'/2 + 1) that can be nonzero when
-// their transform reads no others.  PBI4_MAPS_C2R=rocfft keeps the rocFFT routes (comparisons
-// and timing).
+// On the routes of the library's own the fused filter writes only the columns x' < C =
+// min(R + 1, N'/2 + 1) and, per shell, only the lines (x', q_y) inside the shell's disk
+// x'^2 + q_y^2 <= largest d2: the others are zero, and the transforms take them as zero without
+// reading them.  PBI4_MAPS_C2R=rocfft keeps the rocFFT routes (comparisons and timing).
 enum class MapsPath { Rocfft3d, Rocfft2d, LdsPlane, TwoPass };
 struct MapsRoute {
     MapsPath path = MapsPath::Rocfft3d;
     int ld = 0;          // row stride of the half spectra (complex entries)
     int cols = 0;        // columns the fused filter writes (0 = all N'/2 + 1)
+    int disk_only = 0;   // the fused filter writes only the lines inside each shell's disk
     C2r2dPlan plane{};   // LdsPlane
 };
 
@@ -437,6 +439,9 @@ MapsRoute maps_route(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
         if (ours && make_c2r2d_plan(Np, C, r.ld, Kstride, r.plane)) {
             r.path = MapsPath::LdsPlane;
             r.cols = C;
+            r.disk_only = 1;
+            r.plane.nb = sh.nb;
+            r.plane.shell_d2 = sh.d2range_d;
         } else {
             r.path = MapsPath::Rocfft2d;
         }
@@ -445,6 +450,7 @@ MapsRoute maps_route(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
     if (ours && Np == grid && sh.contiguous && 2 * sh.R < Np && maps_two_pass_supported(Np)) {
         r.path = MapsPath::TwoPass;
         r.cols = C;
+        r.disk_only = 1;
         r.ld = (C + 15) / 16 * 16;   // whole 16-column blocks for the column pass
         return r;
     }
@@ -461,6 +467,9 @@ int maps_ld(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
 }
 int maps_cols(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
     return maps_route(Np, grid, sh, Kstride).cols;
+}
+int maps_disk_only(int Np, int grid, const ShellInfo &sh, int64_t Kstride) {
+    return maps_route(Np, grid, sh, Kstride).disk_only;
 }
 
 // Full-grid maps: one 3D c2r execution transforms at most this many real outputs (all its
@@ -505,8 +514,8 @@ void maps_c2r(pbi4_ctx *ctx, const ShellInfo &sh, int Np, int grid, int nmaps, i
     }
     if (r.path == MapsPath::TwoPass) {
         TimedScope ts(&ctx->timer, "pbi4_c2r", s);
-        launch_maps_two_pass(Np, r.cols, r.ld, nmaps, Kstride, get_twiddle(ctx, Np), spectra, maps,
-                             s);
+        launch_maps_two_pass(Np, r.cols, r.ld, nmaps, sh.nb, sh.d2range_d, Kstride,
+                             get_twiddle(ctx, Np), spectra, maps, s);
         return;
     }
     FftPlan &p = maps_plan(ctx, r, Np, nmaps, Kstride);
@@ -759,6 +768,7 @@ void bk_from_shells(pbi4_ctx *ctx, const pbi4_bk_desc *d, BkLayout &L, ShellInfo
     f.shell_d2 = sh.d2range_d;
     f.ld = maps_ld(L.Np, d->grid, sh, L.Kstride);
     f.cols = maps_cols(L.Np, d->grid, sh, L.Kstride);
+    f.disk_only = maps_disk_only(L.Np, d->grid, sh, L.Kstride);
     f.f2 = static_cast<const float2 *>(delta2);
     f.f4 = static_cast<const float2 *>(delta4);
     PBI4_CHECK(d->multipoles || (!delta2 && !delta4), PBI4_EINVAL,
@@ -1194,6 +1204,7 @@ int pbi4_measure_bk_cross(pbi4_ctx *ctx, const pbi4_bk_desc *d, const float *cou
             fp.shell_d2 = sh.d2range_d;
             fp.ld = maps_ld(L.Np, d->grid, sh, L.Kstride);
             fp.cols = maps_cols(L.Np, d->grid, sh, L.Kstride);
+            fp.disk_only = maps_disk_only(L.Np, d->grid, sh, L.Kstride);
             TimedScope ts(&ctx->timer, "pbi4_filter", s);
             if (maps_fused(L.Np, d->grid, sh, L.Kstride))
                 launch_filter_z(fp, static_cast<const float2 *>(fields[f]), L.shell_sum + f * nb,

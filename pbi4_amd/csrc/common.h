@@ -116,6 +116,8 @@ struct FilterPlan {
     int ld;                   // the fused z filter's output row stride (>= N'/2 + 1)
     const float2 *f2, *f4;    // explicit delta_2, delta_4 fields, or null for L_l(mu) delta
     int cols;                 // fused z filter: columns x' < cols written (0 = all N'/2 + 1)
+    int disk_only;            // fused z filter: lines (x', q_y) with x'^2 + q_y^2 above the shell's
+                              // largest d2 (zero) are not written; their readers take them as zero
 };
 void launch_filter(const FilterPlan &f, const float2 *delta, const double *shell_sum,
                    const double *nmodes, float2 *spectra, float *scale_exp, hipStream_t s);
@@ -133,6 +135,9 @@ struct C2r2dPlan {
     int64_t in_plane;         // complex entries per input plane (N' ld)
     int64_t Kstride;          // floats per output map
     size_t lds;               // dynamic LDS bytes
+    int nb;                   // shells: map m holds shell m mod nb
+    const int32_t *shell_d2;  // per shell: smallest and largest d2; lines (x', q_y) with
+                              // x'^2 + q_y^2 above the largest are zero and are not read
 };
 // False when N' has no compiled transform or the planes do not fit LDS.
 bool make_c2r2d_plan(int Np, int C, int ld, int64_t Kstride, C2r2dPlan &p);
@@ -141,8 +146,9 @@ void launch_c2r2d(const C2r2dPlan &p, const float2 *spectra, const float2 *tw, f
 // Full-grid maps (N' = N in 128, 192, 256, 384): after the fused z filter, the inverse DFTs along
 // y of the columns x' < C in place (row stride ld), then each row's c2r into the maps.
 bool maps_two_pass_supported(int Np);
-void launch_maps_two_pass(int Np, int C, int ld, int nmaps, int64_t Kstride, const float2 *tw,
-                          float2 *spectra, float *maps, hipStream_t s);
+void launch_maps_two_pass(int Np, int C, int ld, int nmaps, int nb, const int32_t *shell_d2,
+                          int64_t Kstride, const float2 *tw, float2 *spectra, float *maps,
+                          hipStream_t s);
 
 // ---------------------------------------------------------------- triangles (triangle.hip)
 struct GemmPlan {

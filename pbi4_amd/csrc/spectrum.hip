@@ -461,6 +461,7 @@ __global__ __launch_bounds__(256) void k_filter_z(FilterPlan f, const float2 *de
         for (int j = 0; j < ZC; ++j) acc[l][j] = make_float2(0.0f, 0.0f);
     int qlo = 0, qhi = -1;
     const int r2 = x * x + qy * qy;
+    if (f.disk_only && r2 > f.shell_d2[2 * (s - 1) + 1]) return;   // zero, and never read
     if (x <= R && qy <= R && qy >= -R) {
         const int a = f.shell_d2[2 * (s - 1)], b = f.shell_d2[2 * (s - 1) + 1];
         if (a <= b && b >= r2) {
@@ -763,17 +764,22 @@ __global__ __launch_bounds__(256) void k_c2r2d(C2r2dPlan p, const float2 *spec, 
     static_assert(A1 * B1 == NP && 2 * A2 * B2 == NP, "factors of N' and N'/2");
     constexpr int M = NP / 2, CS = NP + 1;
     extern __shared__ float2 sm[];
-    const int C = p.C;
-    float2 *twl = sm, *col = sm + NP, *row = col + C * CS;
+    float2 *twl = sm, *col = sm + NP, *row = col + p.C * CS;
     const int plane = blockIdx.x;
     const int map = plane / NP, z = plane - map * NP;
+    // the shell's disk: lines with x'^2 + q_y^2 > b are zero (the filter did not write them)
+    const int b = p.shell_d2[2 * (map % p.nb) + 1];
+    const int C = b < 0 ? 0 : min(p.C, isqrt_floor(b) + 1);
     for (int i = threadIdx.x; i < NP; i += blockDim.x) twl[i] = tw[i];
     const float2 *in = spec + (int64_t)plane * p.in_plane;
     for (int t = threadIdx.x; t < C * B1; t += blockDim.x) {
         const int j = t / C, c = t - j * C;
         float2 x[A1];
 #pragma unroll
-        for (int m = 0; m < A1; ++m) x[m] = in[(int64_t)(j + B1 * m) * p.ld + c];
+        for (int m = 0; m < A1; ++m) {
+            const int q = j + B1 * m, qy = q <= NP / 2 ? q : q - NP;
+            x[m] = c * c + qy * qy <= b ? in[(int64_t)q * p.ld + c] : make_float2(0.0f, 0.0f);
+        }
         rdft<A1>(x);
         float2 *cc = col + c * CS + j;
 #pragma unroll
@@ -836,21 +842,32 @@ __global__ __launch_bounds__(256) void k_c2r2d(C2r2dPlan p, const float2 *spec, 
 // place: a work-group owns whole columns).
 constexpr int kColBlock = 16;
 template <int NP, int A, int B>
-__global__ __launch_bounds__(256) void k_maps_cols(int C, int ld, const float2 *tw, float2 *spec) {
+__global__ __launch_bounds__(256) void k_maps_cols(int C, int ld, int nb, const int32_t *shell_d2,
+                                                   const float2 *tw, float2 *spec) {
     static_assert(A * B == NP, "factors of N");
     constexpr int CS = NP + 1;
     __shared__ float2 twl[NP];
     __shared__ float2 col[kColBlock * CS];
     const int cb0 = blockIdx.y * kColBlock;
+    // the shell's disk: columns x' with x'^2 > b, and lines with x'^2 + q_y^2 > b, are zero (the
+    // filter did not write them; the row pass does not read those columns)
+    const int b = shell_d2[2 * ((int)(blockIdx.x / NP) % nb) + 1];
+    if (b < 0) return;
+    C = min(C, isqrt_floor(b) + 1);
+    if (cb0 >= C) return;
     float2 *pl = spec + (int64_t)blockIdx.x * NP * ld;
     for (int i = threadIdx.x; i < NP; i += blockDim.x) twl[i] = tw[i];
     __syncthreads();
     for (int t = threadIdx.x; t < kColBlock * B; t += blockDim.x) {
         const int j = t / kColBlock, c = t - j * kColBlock;
         if (cb0 + c >= C) continue;
+        const int x2 = (cb0 + c) * (cb0 + c);
         float2 x[A];
 #pragma unroll
-        for (int m = 0; m < A; ++m) x[m] = pl[(int64_t)(j + B * m) * ld + cb0 + c];
+        for (int m = 0; m < A; ++m) {
+            const int q = j + B * m, qy = q <= NP / 2 ? q : q - NP;
+            x[m] = x2 + qy * qy <= b ? pl[(int64_t)q * ld + cb0 + c] : make_float2(0.0f, 0.0f);
+        }
         rdft<A>(x);
         float2 *cc = col + c * CS + j;
 #pragma unroll
@@ -876,7 +893,8 @@ __global__ __launch_bounds__(256) void k_maps_cols(int C, int ld, const float2 *
 // map p / N at z N^2 + y N.
 constexpr int kRowBlock = 8;
 template <int NP, int A2, int B2>
-__global__ __launch_bounds__(256) void k_maps_rows(int C, int ld, int64_t nrows, int64_t Kstride,
+__global__ __launch_bounds__(256) void k_maps_rows(int C, int ld, int64_t nrows, int nb,
+                                                   const int32_t *shell_d2, int64_t Kstride,
                                                    const float2 *tw, const float2 *spec,
                                                    float *maps) {
     static_assert(2 * A2 * B2 == NP, "factors of N/2");
@@ -890,13 +908,16 @@ __global__ __launch_bounds__(256) void k_maps_rows(int C, int ld, int64_t nrows,
         const int r = t / B2, j = t - r * B2;
         if (r0 + r >= nrows) continue;
         const float2 *X = spec + (r0 + r) * ld;     // rows of all planes are ld apart
+        // columns beyond the shell's disk are zero (the column pass did not write them)
+        const int b = shell_d2[2 * ((int)((r0 + r) / ((int64_t)NP * NP)) % nb) + 1];
+        const int Cs = b < 0 ? 0 : min(C, isqrt_floor(b) + 1);
         float2 x[A2];
 #pragma unroll
         for (int m = 0; m < A2; ++m) {
             const int k = j + B2 * m;
             float2 xk = make_float2(0.0f, 0.0f), xm = xk;
-            if (k < C) xk = X[k];
-            if (M - k < C) xm = conjf2(X[M - k]);
+            if (k < Cs) xk = X[k];
+            if (M - k < Cs) xm = conjf2(X[M - k]);
             x[m] = cadd(cadd(xk, xm), times_i(cmul(csub(xk, xm), twl[k])));
         }
         rdft<A2>(x);
@@ -1105,34 +1126,24 @@ void launch_c2r2d(const C2r2dPlan &p, const float2 *spectra, const float2 *tw, f
 
 bool maps_two_pass_supported(int Np) { return Np == 128 || Np == 192 || Np == 256 || Np == 384; }
 
-void launch_maps_two_pass(int Np, int C, int ld, int nmaps, int64_t Kstride, const float2 *tw,
-                          float2 *spectra, float *maps, hipStream_t s) {
+void launch_maps_two_pass(int Np, int C, int ld, int nmaps, int nb, const int32_t *shell_d2,
+                          int64_t Kstride, const float2 *tw, float2 *spectra, float *maps,
+                          hipStream_t s) {
     PBI4_CHECK(maps_two_pass_supported(Np) && C >= 1 && C <= Np / 2 && ld >= C, PBI4_EINVAL,
                "full-grid maps transform: unsupported size");
     const dim3 gc((unsigned)((int64_t)nmaps * Np), (unsigned)ceil_div(C, kColBlock));
     const int64_t nrows = (int64_t)nmaps * Np * Np;
     const dim3 gr((unsigned)ceil_div(nrows, kRowBlock));
+    auto run = [&](auto kc, auto kr) {
+        hipLaunchKernelGGL(kc, gc, dim3(256), 0, s, C, ld, nb, shell_d2, tw, spectra);
+        hipLaunchKernelGGL(kr, gr, dim3(256), 0, s, C, ld, nrows, nb, shell_d2, Kstride, tw,
+                           spectra, maps);
+    };
     switch (Np) {
-        case 128:
-            hipLaunchKernelGGL((k_maps_cols<128, 8, 16>), gc, dim3(256), 0, s, C, ld, tw, spectra);
-            hipLaunchKernelGGL((k_maps_rows<128, 8, 8>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
-                               tw, spectra, maps);
-            break;
-        case 192:
-            hipLaunchKernelGGL((k_maps_cols<192, 12, 16>), gc, dim3(256), 0, s, C, ld, tw, spectra);
-            hipLaunchKernelGGL((k_maps_rows<192, 8, 12>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
-                               tw, spectra, maps);
-            break;
-        case 256:
-            hipLaunchKernelGGL((k_maps_cols<256, 16, 16>), gc, dim3(256), 0, s, C, ld, tw, spectra);
-            hipLaunchKernelGGL((k_maps_rows<256, 8, 16>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
-                               tw, spectra, maps);
-            break;
-        default:
-            hipLaunchKernelGGL((k_maps_cols<384, 16, 24>), gc, dim3(256), 0, s, C, ld, tw, spectra);
-            hipLaunchKernelGGL((k_maps_rows<384, 12, 16>), gr, dim3(256), 0, s, C, ld, nrows, Kstride,
-                               tw, spectra, maps);
-            break;
+        case 128: run(k_maps_cols<128, 8, 16>, k_maps_rows<128, 8, 8>); break;
+        case 192: run(k_maps_cols<192, 12, 16>, k_maps_rows<192, 8, 12>); break;
+        case 256: run(k_maps_cols<256, 16, 16>, k_maps_rows<256, 8, 16>); break;
+        default: run(k_maps_cols<384, 16, 24>, k_maps_rows<384, 12, 16>); break;
     }
     PBI4_HIP(hipGetLastError());
 }
@@ -1149,11 +1160,12 @@ void launch_filter_z(const FilterPlan &f, const float2 *delta, const double *she
     PBI4_CHECK(f.ld >= (f.cols ? f.cols : f.Np / 2 + 1), PBI4_EINVAL,
                "the fused z filter's row stride is too small");
     const int lines = f.Np * (f.cols ? f.cols : f.Np / 2 + 1);
-    // planes per thread: 20 with three multipole maps (120 accumulator registers), else 25
+    // planes per thread: 8 with three multipole maps, else 25 (fewer planes per thread, more
+    // threads in flight: the mode loads of a line's runs are what a thread waits on)
     if (f.nell == 3)
-        hipLaunchKernelGGL((k_filter_z<20, 3>),
+        hipLaunchKernelGGL((k_filter_z<8, 3>),
                            dim3((unsigned)ceil_div(lines, 256), (unsigned)f.nshell,
-                                (unsigned)ceil_div(f.Np, 20)),
+                                (unsigned)ceil_div(f.Np, 8)),
                            dim3(256), 0, s, f, delta, scale_exp, tw, spectra);
     else
         hipLaunchKernelGGL((k_filter_z<25, 1>),

@@ -11,8 +11,8 @@ transforms of the same spectra, so the bispectrum must agree to FP32 rounding: r
 involve the maps (k's, shell powers, shot noise, N_tr) bitwise, B, B_2, B_4 within 3e-5
 (band-limited) or 1e-4 (full grid) of max(|B_l|, |B_0|, sigma_B) (sigma_B =
 sqrt(P1 P2 P3 / (N_tr kf^3)), the estimator's Gaussian standard deviation), in FP32 sums so the
-comparison sees the maps only.  Measured: 1.2e-6 to 1.3e-5 on the band-limited grids; 3.4e-6,
-1.3e-5, 1.6e-5 and 4.0e-5 on the full 128^3, 128^3, 192^3 and 256^3 grids, where the sums run over
+comparison sees the maps only.  Measured: 1.2e-6 to 1.3e-5 on the band-limited grids; 9.4e-6,
+3.4e-6, 1.8e-5 and 5.4e-5 on the full 128^3, 128^3, 192^3 and 256^3 grids, where the sums run over
 up to 1.7e7 points of strongly cancelling triple products (the FP32 sums of two exact transforms'
 roundings).  Sizes without a transform of the library's own must give bitwise the rocFFT result.
 Against the FP64 oracle the default routes are held to 1e-4 by the parity and configuration
