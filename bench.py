@@ -86,7 +86,8 @@ def parse():
     ap.add_argument("--deadline", type=float, default=530.0,
                     help="seconds from process start by which the run must end")
     ap.add_argument("--gemm-only", type=int, default=0,
-                    help="run only the triangle GEMM timing (for profiling), print it and exit")
+                    help="run only the triangle GEMM timing (for profiling), print it and exit: "
+                         "1 = 48 shells on 150^3, 2 = the full-grid 384^3 / 256^3 shapes")
     ap.add_argument("--cfg4-only", type=int, default=0,
                     help="run only the cfg4 stream, print it and exit")
     ap.add_argument("--large-only", type=int, default=0,
@@ -246,6 +247,9 @@ def main():
 
     torch.cuda.init()
     ctx = Context(0)
+    if args.gemm_only == 2:
+        print(json.dumps({"triangle_gemm_full_grid": gemm_full_grid(ctx, torch)}), flush=True)
+        return 0
     if args.gemm_only:
         print(json.dumps({"triangle_gemm": gemm_bench(ctx, torch)}), flush=True)
         return 0
@@ -943,6 +947,47 @@ def gemm_bench(ctx, torch):
         out[f"nb48_nell{nell}_K150^3"] = res
         del maps
     log(f"triangle GEMM alone: {out}")
+    return out
+
+
+def gemm_full_grid(ctx, torch):
+    """The bispectrum's triangle GEMM on the full-grid aliasing shapes of the mixed-grid stream
+    at the reference's shell counts (kbin = kcenter = 2, open triangles, redshift space, three
+    multipoles, split-BF16x3): 384^3 with 64 shells (25 424 triangles) and 256^3 with 42 (7 714),
+    timed inside pbi4_measure_bk on a random half spectrum (the GEMM does not depend on the
+    values).  mfma_frac = issued MFMA work over the bf16 peak."""
+    from pbi4_amd import _lib
+    out = {}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    kf = np.float32(2 * np.pi / BOX)
+    for N, nb in ((384, 64), (256, 42)):
+        itot = len(triangle_list(nb, 2.0, 2.0, True))
+        ntr = ctx.count_triangles(nb, N, 2.0, 2.0, True, False, itot)
+        counts = torch.from_numpy((ntr.astype(np.float64) * float(N) ** 3).astype(np.float32)).cuda()
+        delta = torch.randn((N, N, N // 2 + 1), generator=g, device="cuda", dtype=torch.complex64)
+        desc = ctx.bk_desc(nb, N, 2.0, 2.0, True, kf, itot, 10 ** 8, True, (0.0, 0.0, 1.0),
+                           _lib.PREC_BF16X3, 0)
+        assert ctx.maps_grid(desc) == N
+        ws = ctx._ws(ctx.bk_workspace(desc))
+        ctx.bk(delta, counts, desc, ws=ws)
+        torch.cuda.synchronize()
+        ctx.timing_enable("pbi4_triangle_gemm")
+        reps = 3
+        for _ in range(reps):
+            ctx.bk(delta, counts, desc, ws=ws)
+        torch.cuda.synchronize()
+        ms, k = ctx.timing_read()
+        ctx.timing_enable(None)
+        avg = ms / max(k, 1)
+        tiles = gemm_blocks(nb, 3, 2.0, 2.0, True)
+        ex = 3 * 2.0 * 256 * tiles["block_sparse"] * N ** 3 / (avg * 1e-3) / 1e12
+        out[f"{N}^3_nb{nb}_nell3"] = {"avg_ms": round(avg, 3), "executed_tflops": round(ex, 1),
+                                      "mfma_frac": round(ex / BF16_PEAK_TFLOPS, 4),
+                                      "tiles_issued_of_dense": round(tiles["block_sparse"] / tiles["dense"], 4)}
+        log(f"full-grid GEMM {N}^3 nb={nb}: {out[f'{N}^3_nb{nb}_nell3']}")
+        del ws, delta, counts
+        torch.cuda.empty_cache()
     return out
 
 
