@@ -126,19 +126,21 @@ __device__ __forceinline__ int tile_of(const PosXform &x, int T, int NT, const R
     const float invT = 1.0f / (float)T;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        // c / T for 0 <= c < 2^23 from the float reciprocal, corrected by one either way
+        // c / T for 0 <= c < 2^23 from the float reciprocal, corrected by one either way (24-bit
+        // products: exact for these operands, and full rate)
         int q = (int)((float)c[d] * invT);
-        if (q * T > c[d]) --q;
-        else if ((q + 1) * T <= c[d]) ++q;
+        if (__mul24(q, T) > c[d]) --q;
+        else if (__mul24(q + 1, T) <= c[d]) ++q;
         t[d] = q;
         if (packed) {
-            uint32_t loc = (uint32_t)(c[d] - t[d] * T);
+            uint32_t loc = (uint32_t)(c[d] - __mul24(t[d], T));
             uint32_t fr = (uint32_t)(f[d] * 268435456.0f);   // 2^28
             if (fr > 0x0FFFFFFFu) fr = 0x0FFFFFFFu;
             packed[d] = (loc << 28) | fr;
         }
     }
-    return (t[2] * NT + t[1]) * NT + t[0];
+    // NT^2 < 2^23 for any grid below 46 000 cells, and the tile count fits 31 bits
+    return __mul24(__mul24(t[2], NT) + t[1], NT) + t[0];
 }
 
 // ------------------------------------------------------------------ counting sort
@@ -470,14 +472,16 @@ __device__ __forceinline__ int region_class(int a, int T) { return a < kHalo ? 0
 __device__ __forceinline__ int class_start(int c, int T) { return c == 0 ? 0 : c == 1 ? kHalo : T + kHalo; }
 __device__ __forceinline__ int class_extent(int c, int T) { return c == 1 ? T : kHalo; }
 
+// Every factor is below R <= 20 and every product below R^3: the 24-bit multiplies (full rate,
+// where a 32-bit v_mul_lo issues at a quarter of it) are exact.
 __device__ __forceinline__ int region_index(int a, int b, int c, int T, int R) {
     const int ka = region_class(a, T), kb = region_class(b, T), kc = region_class(c, T);
     const int sa = class_start(ka, T), sb = class_start(kb, T), sc = class_start(kc, T);
     const int ea = class_extent(ka, T), eb = class_extent(kb, T), ec = class_extent(kc, T);
     // pieces before (kc, kb, ka): all of the lower z classes, then the lower y classes of this z
     // class, then the lower x classes of this (z, y) class
-    const int off = sc * R * R + ec * (sb * R + eb * sa);
-    return off + ((c - sc) * eb + (b - sb)) * ea + (a - sa);
+    const int off = __mul24(__mul24(sc, R), R) + __mul24(ec, __mul24(sb, R) + __mul24(eb, sa));
+    return off + __mul24(__mul24(c - sc, eb) + (b - sb), ea) + (a - sa);
 }
 
 
@@ -956,9 +960,11 @@ void k_gather(int N, int T, int NT, int R, int ngrids, const float *regions, flo
     // (a guarded load becomes a branch with a wait after it)
     for (int o = 0; o < nx * ny; ++o) {
         const int ox = o % nx, oy = o / nx;
-        const int64_t tb = ((int64_t)tz * NT + nty[oy]) * NT + ntx[ox];
-        const int64_t tm = ((int64_t)tzm * NT + nty[oy]) * NT + ntx[ox];
-        const int64_t tp = ((int64_t)tzp * NT + nty[oy]) * NT + ntx[ox];
+        // tile numbers as tile_of forms them (24-bit products, below 2^31 tiles)
+        const int yx = __mul24(nty[oy], NT) + ntx[ox];
+        const int64_t tb = __mul24(__mul24(tz, NT), NT) + yx;
+        const int64_t tm = __mul24(__mul24(tzm, NT), NT) + yx;
+        const int64_t tp = __mul24(__mul24(tzp, NT), NT) + yx;
         const float *r = regions + (tb * ngrids + g) * R3;
         const float *rm = regions + (tm * ngrids + g) * R3;
         const float *rp = regions + (tp * ngrids + g) * R3;
