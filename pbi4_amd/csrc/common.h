@@ -166,6 +166,10 @@ struct GemmPlan {
     int cfw = 0;        // widest window, rounded up to a compiled width
     int maxw = 0;       // widest window (column fragments): partial rows are 16 maxw long
     int ucap = 0;       // most column fragments one work-group stages (union of its 4 waves)
+    // launches, one per compiled window width: row blocks rbs[cls_off[k] .. cls_off[k + 1]) on
+    // the cls_w[k]-tile body, staging at most cls_ucap[k] column fragments
+    std::vector<uint8_t> rbs;
+    std::vector<int> cls_w, cls_off, cls_ucap;
     int moff = 0, mdiv = 1, ncol_l = 0;
     // rows: 0 = pairs (i <= j) at j (j + 1) / 2 + i; 1 = real-space rows (j, l) of the triangle
     // loop, ordered by l then j from max(1, ceil((l + 1 - ish) / 2)), columns = the leg i

@@ -273,6 +273,7 @@ constexpr int kMaxWinWaves = 1024;
 struct GemmWindows {
     uint8_t start[kMaxWinWaves];       // first logical column fragment of each wave's window
     uint8_t width[kMaxWinWaves / 4];   // W of each work-group (0: no triangle reads its rows)
+    uint8_t rbs[kMaxWinWaves / 4];     // row blocks in launch order (launch k: a range of them)
 };
 
 constexpr int kWinKT = 32;         // x-values per LDS stage (one bf16 MFMA k-step)
@@ -486,17 +487,19 @@ __device__ __forceinline__ void win_body(const float *maps, int64_t K, int64_t K
 }
 
 // Dispatches each work-group on its window width W (1 .. CFW), so that the MFMA loop of every
-// width is straight-line code; the kernel's registers are those of the widest case.
+// width is straight-line code; the kernel's registers are those of the widest case, so the host
+// launches the row blocks of each width class on the narrowest body that holds them.  This
+// launch covers the row blocks win.rbs[rb_off .. rb_off + nrb_l).
 template <int PREC, int CFW, int RF>
 __global__ __launch_bounds__(kGemmThreads) void k_triangle_gemm_win(
     const float *maps, int64_t K, int64_t Kstride, int nb, int moff, int mdiv, int ncol, int npair,
-    int kchunk, int nrowblocks, int nchunks, int ucap, int nw, int rowmode, int ish, GemmWindows win,
-    float *partials) {
+    int kchunk, int nrowblocks, int nchunks, int ucap, int nw, int rowmode, int ish, int rb_off,
+    int nrb_l, GemmWindows win, float *partials) {
     constexpr int NVB = WinTile<CFW>::NVB;
     const int w = blockIdx.x;
     const int q = w / kXcds;
-    const int rowblock = q % nrowblocks;
-    const int chunk = (w % kXcds) + kXcds * (q / nrowblocks);
+    const int rowblock = win.rbs[rb_off + q % nrb_l];
+    const int chunk = (w % kXcds) + kXcds * (q / nrb_l);
     if (chunk >= nchunks) return;
     const int W = win.width[rowblock];
 #define PBI4_WIN_CASE(n)                                                                       \
@@ -782,6 +785,7 @@ static bool plan_windows(GemmPlan &g, const std::vector<int32_t> &tri, const Ent
     // own first one, moved left where the window would pass the last logical fragment
     const int cfl = (int)ceil_div(g.ncol_l, 16);
     std::vector<uint8_t> start(nwaves, 0), width(g.nrowblocks, 0);
+    std::vector<int> span(g.nrowblocks, 0);
     int maxw = 0, ucap = 0;
     for (int b = 0; b < g.nrowblocks; ++b) {
         int W = 0, first = -1;
@@ -800,6 +804,7 @@ static bool plan_windows(GemmPlan &g, const std::vector<int32_t> &tri, const Ent
             u1 = std::max<int>(u1, start[w] + W);
         }
         width[b] = (uint8_t)W;
+        span[b] = u1 - u0;
         maxw = std::max(maxw, W);
         ucap = std::max(ucap, u1 - u0);
     }
@@ -811,17 +816,56 @@ static bool plan_windows(GemmPlan &g, const std::vector<int32_t> &tri, const Ent
         }
     if (maxw == 0) return why = "no triangles", false;
     if (cfw == 0) return why = "a window wider than 8 column tiles", false;
-    const int ucap_max = cfw == 2   ? WinTile<2>::UCAP
-                         : cfw == 3 ? WinTile<3>::UCAP
-                         : cfw == 4 ? WinTile<4>::UCAP
-                         : cfw == 6 ? WinTile<6>::UCAP
-                                    : WinTile<8>::UCAP;
-    if (ucap > ucap_max) return why = "a work-group's windows span too many column tiles", false;
+    auto ucap_of = [](int c) {
+        return c == 2   ? WinTile<2>::UCAP
+               : c == 3 ? WinTile<3>::UCAP
+               : c == 4 ? WinTile<4>::UCAP
+               : c == 6 ? WinTile<6>::UCAP
+                        : WinTile<8>::UCAP;
+    };
+    if (ucap > ucap_of(cfw)) return why = "a work-group's windows span too many column tiles", false;
     g.cfw = cfw;
     g.maxw = maxw;
     g.ucap = ucap;
     g.wstart = std::move(start);
     g.wwidth = std::move(width);
+    // launch classes: each row block on the narrowest compiled body that holds its window and
+    // staged span (the 2- and 4-tile bodies hold 103 and 142 registers against 220 for the 8-tile
+    // one: 4 and 3 waves per SIMD instead of 2, and less LDS).  Separate launches also stop the
+    // row blocks of one K chunk from sharing its maps in L2, which outweighs the occupancy from
+    // five row blocks on (split / one launch: 256^3 with 42 shells x 3, 4 row blocks: 7.8 / 9.5
+    // ms; 192^3 with 32 x 3, 5: 2.48 / 2.61; 150^3 with 48 x 3, 5: 2.21 / 2.03; 384^3 with
+    // 64 x 3, 17: 78.0 / 74.1), so only plans of at most four row blocks on 2^21 or more points
+    // split.
+    const int widths[5] = {2, 3, 4, 6, 8};
+    std::vector<std::vector<int>> by(5);
+    const bool split = g.K >= (int64_t)1 << 21 && g.nrowblocks <= 4;
+    for (int b = 0; b < g.nrowblocks; ++b) {
+        int k = 4;
+        while (widths[k] != cfw) --k;                  // the widest body: holds every row block
+        if (split)
+            for (int c = 0; c < 5 && widths[c] <= cfw; ++c)
+                if (widths[c] >= g.wwidth[b] && ucap_of(widths[c]) >= span[b]) {
+                    k = c;
+                    break;
+                }
+        by[k].push_back(b);
+    }
+    g.rbs.clear();
+    g.cls_w.clear();
+    g.cls_off.assign(1, 0);
+    g.cls_ucap.clear();
+    for (int k = 0; k < 5; ++k) {
+        if (by[k].empty()) continue;
+        int u = 1;
+        for (int b : by[k]) {
+            g.rbs.push_back((uint8_t)b);
+            u = std::max(u, span[b]);
+        }
+        g.cls_w.push_back(widths[k]);
+        g.cls_ucap.push_back(u);
+        g.cls_off.push_back((int)g.rbs.size());
+    }
     return true;
 }
 
@@ -904,40 +948,49 @@ static GemmWindows windows_arg(const GemmPlan &g) {
     GemmWindows w{};
     std::copy(g.wstart.begin(), g.wstart.end(), w.start);
     std::copy(g.wwidth.begin(), g.wwidth.end(), w.width);
+    std::copy(g.rbs.begin(), g.rbs.end(), w.rbs);
     return w;
 }
 
+// Launch k of the block-sparse plan (its row blocks on the CFW-tile body).
 template <int PREC, int CFW, int RF>
-static void launch_win(const GemmPlan &g, const float *maps, float *partials, hipStream_t s) {
-    const size_t lds = 2 * win_stage_bytes<PREC>(g.nb, g.ucap);
+static void launch_win(const GemmPlan &g, int k, const GemmWindows &w, const float *maps,
+                       float *partials, hipStream_t s) {
+    const int ucap = g.cls_ucap[k], rb_off = g.cls_off[k], nrb = g.cls_off[k + 1] - rb_off;
+    PBI4_CHECK(ucap <= WinTile<CFW>::UCAP, PBI4_EUNSUPPORTED, "triangle GEMM window class");
+    const size_t lds = 2 * win_stage_bytes<PREC>(g.nb, ucap);
     PBI4_CHECK(lds <= 160 * 1024, PBI4_EUNSUPPORTED, "triangle GEMM tile does not fit in LDS");
     auto kern = k_triangle_gemm_win<PREC, CFW, RF>;
     if (lds > 64 * 1024)
         PBI4_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds));
-    const unsigned nwg = (unsigned)((int64_t)g.nrowblocks * g.nkchunks_pad);
+    const unsigned nwg = (unsigned)((int64_t)nrb * g.nkchunks_pad);
     hipLaunchKernelGGL(kern, dim3(nwg), dim3(kGemmThreads), lds, s, maps, g.K, g.Kstride, g.nb,
                        g.moff, g.mdiv, g.ncol_l, g.npair, g.kchunk, g.nrowblocks, g.nkchunks,
-                       g.ucap, g.maxw * 16, g.rowmode, g.ish, windows_arg(g), partials);
+                       ucap, g.maxw * 16, g.rowmode, g.ish, rb_off, nrb, w, partials);
 }
 
 template <int PREC, int CFW>
-static void launch_win_rf(const GemmPlan &g, const float *maps, float *partials, hipStream_t s) {
+static void launch_win_rf(const GemmPlan &g, int k, const GemmWindows &w, const float *maps,
+                          float *partials, hipStream_t s) {
     switch (g.rf) {
-        case 1: launch_win<PREC, CFW, 1>(g, maps, partials, s); break;
-        case 2: launch_win<PREC, CFW, 2>(g, maps, partials, s); break;
-        default: launch_win<PREC, CFW, 4>(g, maps, partials, s); break;
+        case 1: launch_win<PREC, CFW, 1>(g, k, w, maps, partials, s); break;
+        case 2: launch_win<PREC, CFW, 2>(g, k, w, maps, partials, s); break;
+        default: launch_win<PREC, CFW, 4>(g, k, w, maps, partials, s); break;
     }
 }
 
 template <int PREC>
 static void launch_win_cfw(const GemmPlan &g, const float *maps, float *partials, hipStream_t s) {
-    switch (g.cfw) {
-        case 2: launch_win_rf<PREC, 2>(g, maps, partials, s); break;
-        case 3: launch_win_rf<PREC, 3>(g, maps, partials, s); break;
-        case 4: launch_win_rf<PREC, 4>(g, maps, partials, s); break;
-        case 6: launch_win_rf<PREC, 6>(g, maps, partials, s); break;
-        default: launch_win_rf<PREC, 8>(g, maps, partials, s); break;
+    const GemmWindows w = windows_arg(g);
+    for (size_t k = 0; k < g.cls_w.size(); ++k) {
+        switch (g.cls_w[k]) {
+            case 2: launch_win_rf<PREC, 2>(g, (int)k, w, maps, partials, s); break;
+            case 3: launch_win_rf<PREC, 3>(g, (int)k, w, maps, partials, s); break;
+            case 4: launch_win_rf<PREC, 4>(g, (int)k, w, maps, partials, s); break;
+            case 6: launch_win_rf<PREC, 6>(g, (int)k, w, maps, partials, s); break;
+            default: launch_win_rf<PREC, 8>(g, (int)k, w, maps, partials, s); break;
+        }
     }
 }
 
