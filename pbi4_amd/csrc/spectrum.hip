@@ -953,7 +953,12 @@ __global__ __launch_bounds__(256) void k_maps_rows(int C, int ld, int64_t nrows,
         const int64_t p = g / NP;                    // plane = map N + z
         const int y = (int)(g - p * NP);
         const int64_t map = p / NP, z = p - map * NP;
-        reinterpret_cast<float2 *>(maps + map * Kstride + (z * NP + y) * NP)[n] = row[rw * M + n];
+        // non-temporal: the full-grid maps (0.5 - 43 GB) pass through no cache before the GEMM
+        // reads them (192 maps of 384^3: 15.9 -> 15.5 ms)
+        typedef float f32x2v __attribute__((ext_vector_type(2)));
+        const float2 v = row[rw * M + n];
+        __builtin_nontemporal_store(
+            f32x2v{v.x, v.y}, reinterpret_cast<f32x2v *>(maps + map * Kstride + (z * NP + y) * NP) + n);
     }
 }
 
