@@ -836,10 +836,12 @@ static bool plan_windows(GemmPlan &g, const std::vector<int32_t> &tri, const Ent
     // five row blocks on (split / one launch: 256^3 with 42 shells x 3, 4 row blocks: 7.8 / 9.5
     // ms; 192^3 with 32 x 3, 5: 2.48 / 2.61; 150^3 with 48 x 3, 5: 2.21 / 2.03; 384^3 with
     // 64 x 3, 17: 78.0 / 74.1), so only plans of at most four row blocks on 2^21 or more points
-    // split.
+    // split.  PBI4_TRIANGLE_GEMM_SPLIT=1 / =0 splits always / never (comparisons and tests).
     const int widths[5] = {2, 3, 4, 6, 8};
     std::vector<std::vector<int>> by(5);
-    const bool split = g.K >= (int64_t)1 << 21 && g.nrowblocks <= 4;
+    const char *force_split = std::getenv("PBI4_TRIANGLE_GEMM_SPLIT");
+    const bool split = force_split ? std::strcmp(force_split, "1") == 0
+                                   : g.K >= (int64_t)1 << 21 && g.nrowblocks <= 4;
     for (int b = 0; b < g.nrowblocks; ++b) {
         int k = 4;
         while (widths[k] != cfw) --k;                  // the widest body: holds every row block

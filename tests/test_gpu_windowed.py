@@ -169,3 +169,39 @@ def test_real_space_rows_jl_match_dense(ctx, case, prec):
     tol = 1e-4
     assert e_jl.max() <= tol, f"max {e_jl.max():.3g} at triangle {int(e_jl.argmax())}"
     assert e_d.max() <= tol
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("case", [(96, 48, 1.0, 1.0, True, True), (128, 64, 1.0, 1.0, True, True),
+                                  (96, 48, 1.0, 1.0, True, False), (128, 64, 1.0, 1.0, True, False)],
+                         ids=lambda c: f"N{c[0]}_nb{c[1]}_ell{int(c[5])}")
+def test_width_classes_equal_one_launch(ctx, case, prec):
+    """The block-sparse plan may launch each class of window widths on its own compiled body (the
+    narrowest that holds its row blocks); every row block still runs the MFMA body of its own
+    width, so the sums are bitwise those of the single launch.  With multipoles these shapes have
+    5 and 17 row blocks of pair rows with windows of 6, 5, 4, 2, 1 and 1 to 7 tiles; in real
+    space (j, l) rows with windows of 1 to 4 tiles.  PBI4_TRIANGLE_GEMM_SPLIT=1 launches each
+    width class on its own body; =0 keeps one launch."""
+    N, nb, kb, kc, iopen, mult = case
+    n = 200000
+    print(f"width classes vs one launch: N={N} nb={nb} multipoles={mult} prec={prec}", flush=True)
+    delta = _field(ctx, N, n, 9 + nb)
+    itot = O.itot_count(nb, kb, kc, iopen)
+    ntr = ctx.count_triangles(nb, N, kb, kc, iopen, False, itot)
+    counts = _dev((ntr.astype(np.float64) * N ** 3).astype(np.float32))
+    kf = np.float32(2 * np.pi / 1000.0)
+    desc = ctx.bk_desc(nb, N, kb, kc, iopen, kf, itot, n, mult, (0.0, 0.0, 1.0), prec, 0)
+    out = {}
+    old = os.environ.get("PBI4_TRIANGLE_GEMM_SPLIT")
+    try:
+        for flag in ("1", "0"):
+            os.environ["PBI4_TRIANGLE_GEMM_SPLIT"] = flag
+            out[flag] = ctx.bk(delta, counts, desc).cpu().numpy()
+    finally:
+        if old is None:
+            os.environ.pop("PBI4_TRIANGLE_GEMM_SPLIT", None)
+        else:
+            os.environ["PBI4_TRIANGLE_GEMM_SPLIT"] = old
+    assert np.all(np.isfinite(out["1"]))
+    bad = np.argwhere(out["1"] != out["0"])
+    assert bad.size == 0, f"{len(bad)} entries differ, first {bad[:5].tolist()}"
