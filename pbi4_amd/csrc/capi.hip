@@ -293,6 +293,8 @@ void check_rocfft_cache() {
 // kind 1: c2r of N^3 with `batch` maps, real output per-map distance `ext` floats.
 // kind 2: forward c2c of one N^3 field from planar real / imaginary parts.
 // kind 3: 2D c2r of `batch` N^2 planes (the maps after their z transform).
+// kind 4: forward c2c over (y, x) of `batch` N^2 planes from planar parts (the interlaced density
+//         whose z axis the combine transforms).
 FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
     static RocfftInit init;
     auto key = std::make_tuple(kind * 1000000 + batch, N, (int)ext);
@@ -309,6 +311,15 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
         fft_check(rocfft_plan_description_set_data_layout(
                       desc, rocfft_array_type_complex_planar, rocfft_array_type_complex_interleaved,
                       nullptr, nullptr, 3, strides, dist, 3, strides, dist),
+                  "rocfft_plan_description_set_data_layout");
+    }
+    if (kind == 4) {
+        // forward c2c over (y, x) of `batch` N x N planes given as planar parts
+        size_t strides[2] = {1, (size_t)N};
+        const size_t dist = (size_t)N * N;
+        fft_check(rocfft_plan_description_set_data_layout(
+                      desc, rocfft_array_type_complex_planar, rocfft_array_type_complex_interleaved,
+                      nullptr, nullptr, 2, strides, dist, 2, strides, dist),
                   "rocfft_plan_description_set_data_layout");
     }
     if (kind == 3) {
@@ -331,10 +342,11 @@ FftPlan &get_plan(pbi4_ctx *ctx, int kind, int N, int batch, int64_t ext) {
                   "rocfft_plan_description_set_data_layout");
     }
     fft_check(rocfft_plan_create(&p.plan, rocfft_placement_notinplace,
-                                 kind == 0   ? rocfft_transform_type_real_forward
-                                 : kind == 2 ? rocfft_transform_type_complex_forward
-                                             : rocfft_transform_type_real_inverse,
-                                 rocfft_precision_single, kind == 3 ? 2 : 3, len, (size_t)batch,
+                                 kind == 0                  ? rocfft_transform_type_real_forward
+                                 : (kind == 2 || kind == 4) ? rocfft_transform_type_complex_forward
+                                                            : rocfft_transform_type_real_inverse,
+                                 rocfft_precision_single, (kind == 3 || kind == 4) ? 2 : 3, len,
+                                 (size_t)batch,
                                  desc),
               "rocfft_plan_create");
     rocfft_plan_description_destroy(desc);
@@ -667,10 +679,21 @@ struct DensityWs {
     void *fftwork;
 };
 // Interlaced: one complex FFT of grid1 + i grid2 (the array the reference transforms,
-// pbi4.py:337), the two grids passed as its planar real and imaginary parts; single grid: r2c.
+// pbi4.py:337), the two grids passed as its planar real and imaginary parts; on the grids 128,
+// 192, 256, 384 rocFFT transforms the z planes over (y, x) and the combine kernel does the z axis
+// (PBI4_DENSITY_FFT=rocfft3d keeps rocFFT's 3D transform and the separate combine); single
+// grid: r2c.
+bool density_fused_z(const pbi4_density_desc &d) {
+    const char *e = std::getenv("PBI4_DENSITY_FFT");
+    return d.interlacing && zfft_combine_supported(d.grid) && !(e && std::strcmp(e, "rocfft3d") == 0);
+}
+FftPlan &density_plan(pbi4_ctx *ctx, const pbi4_density_desc &d) {
+    if (!d.interlacing) return get_plan(ctx, 0, d.grid, 1, 0);
+    return density_fused_z(d) ? get_plan(ctx, 4, d.grid, d.grid, 0) : get_plan(ctx, 2, d.grid, 1, 0);
+}
 DensityWs density_layout(pbi4_ctx *ctx, const pbi4_density_desc &d, Arena &a) {
     const int64_t N = d.grid;
-    FftPlan &p = d.interlacing ? get_plan(ctx, 2, d.grid, 1, 0) : get_plan(ctx, 0, d.grid, 1, 0);
+    FftPlan &p = density_plan(ctx, d);
     DensityWs w;
     w.G = a.take<float2>(d.interlacing ? (size_t)(N * N * N) : (size_t)(N * N * (N / 2 + 1)));
     w.fftwork = a.take<char>(std::max<size_t>(p.work, 1));
@@ -926,7 +949,7 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
         Arena a(ws, ws_bytes);
         DensityWs w = density_layout(ctx, *d, a);
         const int64_t V = (int64_t)d->grid * d->grid * d->grid;
-        FftPlan &p = d->interlacing ? get_plan(ctx, 2, d->grid, 1, 0) : get_plan(ctx, 0, d->grid, 1, 0);
+        FftPlan &p = density_plan(ctx, *d);
         {
             TimedScope ts(&ctx->timer, "pbi4_r2c", s);
             fft_exec(p, const_cast<float *>(grids), w.G, w.fftwork, s,
@@ -935,9 +958,14 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
         const double kF3 = kf3_fcomb(d->box);
         const double cf = (d->interlacing ? 1.0 / (4.0 * kF3) : 1.0 / kF3) / d->npart;
         TimedScope ts(&ctx->timer, "pbi4_combine", s);
-        launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order),
-                       d->interlacing ? get_phase(ctx, d->grid) : nullptr, (float)cf, w.G,
-                       static_cast<float2 *>(delta), s);
+        if (density_fused_z(*d))
+            launch_zfft_combine(d->grid, get_window(ctx, d->grid, d->order), get_phase(ctx, d->grid),
+                                get_twiddle(ctx, d->grid), (float)cf, w.G,
+                                static_cast<float2 *>(delta), s);
+        else
+            launch_combine(d->grid, d->interlacing, get_window(ctx, d->grid, d->order),
+                           d->interlacing ? get_phase(ctx, d->grid) : nullptr, (float)cf, w.G,
+                           static_cast<float2 *>(delta), s);
     });
 }
 

@@ -76,6 +76,12 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
 void launch_combine(int N, int interlacing, const float *wtab, const float2 *ph, float cf,
                     const float2 *G1, float2 *delta, hipStream_t s);
 
+// Interlaced density with the FFT's z axis fused into the combine: G holds every z plane of
+// grid1 + i grid2 transformed over (y, x); tw[k] = e^{2 pi i k / N}.  Grids 128, 192, 256, 384.
+bool zfft_combine_supported(int N);
+void launch_zfft_combine(int N, const float *wtab, const float2 *ph, const float2 *tw, float cf,
+                         const float2 *G, float2 *delta, hipStream_t s);
+
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
                        float2 *out, hipStream_t s);
 void launch_qk(int N, int norder, const int *idx, const float2 *F, float2 *out, hipStream_t s);

@@ -14,44 +14,15 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 
 // ------------------------------------------------------------------ fcomb (powerI4.f90:251-378)
 
-// One thread per iteration (ix, iy, iz) in [0, N/2]^3 of the reference's interlaced loop.  The
-// iteration owns the <= 8 cells {ix,icx} x {iy,icy} x {iz,icz}; their eight ordered stores are
-// replayed in registers (a later store to the same cell wins, later stores read earlier ones),
-// then the cells in the x <= N/2 half are written.  Code bits: 1 = icx, 2 = icy, 4 = icz.
-// F is the full N^3 transform of grid1 + i grid2 (one complex FFT).
-// Threads: first the iterations with x in [1, N/2) (x runs fastest), then those with x in
-// {0, N/2}, so the ordered-store replay, needed only where ix = icx, runs in waves of its own.
-// ph[s + N] = e^{-i theta s} for s in [-N, 3N/2]: cm(s) = i e^{-i theta s}, s = ix +- iy +- iz,
-// one table entry (rounded once from FP64) per phase.
-__global__ void k_combine_interlaced(int N, const float2 *ph, const float *wtab, float cf,
-                                     const float2 *F, float2 *delta) {
-    const int h = N / 2 + 1, hin = N / 2 - 1;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nin = (int64_t)hin * h * h;
-    int ix, iy, iz;
-    if (t < nin) {
-        ix = 1 + (int)(t % hin);
-        const int64_t r = t / hin;
-        iy = (int)(r % h);
-        iz = (int)(r / h);
-    } else {
-        const int64_t u = t - nin;
-        if (u >= 2 * (int64_t)h * h) return;
-        ix = (u & 1) ? N / 2 : 0;
-        iy = (int)((u >> 1) % h);
-        iz = (int)((u >> 1) / h);
-    }
+// The octet (ix, iy, iz) of the interlaced combine from its 8 values orig[k] = F(X, Y, Z), X in
+// {ix, icx}, Y in {iy, icy}, Z in {iz, icz} (code bits 1 = icx, 2 = icy, 4 = icz): the cells in
+// the x <= N/2 half are written, replaying the eight ordered stores where cells coincide.
+__device__ __forceinline__ void combine_octet(int N, int ix, int iy, int iz, const float2 *orig,
+                                              const float2 *ph, const float *wtab, float cf,
+                                              float2 *delta) {
+    const int h = N / 2 + 1;
     const int icx = (N - ix) % N, icy = (N - iy) % N, icz = (N - iz) % N;
     const int mask = (ix == icx ? 1 : 0) | (iy == icy ? 2 : 0) | (iz == icz ? 4 : 0);
-
-    // The 8 values F(X, Y, Z), X in {ix, icx}, Y in {iy, icy}, Z in {iz, icz}, loaded as one batch.
-    float2 orig[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int X = (k & 1) ? icx : ix, Y = (k & 2) ? icy : iy, Z = (k & 4) ? icz : iz;
-        orig[k] = F[((int64_t)Z * N + Y) * N + X];
-    }
-
     auto cm = [&](int sidx) {  // i * e^{-i theta s}
         const float2 v = ph[sidx + N];
         return make_float2(-v.y, v.x);
@@ -108,6 +79,44 @@ __global__ void k_combine_interlaced(int N, const float2 *ph, const float *wtab,
         const int Y = (k & 2) ? icy : iy, Z = (k & 4) ? icz : iz;
         delta[((int64_t)Z * N + Y) * h + ix] = get(k);
     }
+}
+
+// One thread per iteration (ix, iy, iz) in [0, N/2]^3 of the reference's interlaced loop.  The
+// iteration owns the <= 8 cells {ix,icx} x {iy,icy} x {iz,icz}; their eight ordered stores are
+// replayed in registers (a later store to the same cell wins, later stores read earlier ones),
+// then the cells in the x <= N/2 half are written.  Code bits: 1 = icx, 2 = icy, 4 = icz.
+// F is the full N^3 transform of grid1 + i grid2 (one complex FFT).
+// Threads: first the iterations with x in [1, N/2) (x runs fastest), then those with x in
+// {0, N/2}, so the ordered-store replay, needed only where ix = icx, runs in waves of its own.
+// ph[s + N] = e^{-i theta s} for s in [-N, 3N/2]: cm(s) = i e^{-i theta s}, s = ix +- iy +- iz,
+// one table entry (rounded once from FP64) per phase.
+__global__ void k_combine_interlaced(int N, const float2 *ph, const float *wtab, float cf,
+                                     const float2 *F, float2 *delta) {
+    const int h = N / 2 + 1, hin = N / 2 - 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nin = (int64_t)hin * h * h;
+    int ix, iy, iz;
+    if (t < nin) {
+        ix = 1 + (int)(t % hin);
+        const int64_t r = t / hin;
+        iy = (int)(r % h);
+        iz = (int)(r / h);
+    } else {
+        const int64_t u = t - nin;
+        if (u >= 2 * (int64_t)h * h) return;
+        ix = (u & 1) ? N / 2 : 0;
+        iy = (int)((u >> 1) % h);
+        iz = (int)((u >> 1) / h);
+    }
+    const int icx = (N - ix) % N, icy = (N - iy) % N, icz = (N - iz) % N;
+    // The 8 values F(X, Y, Z), X in {ix, icx}, Y in {iy, icy}, Z in {iz, icz}, loaded as one batch.
+    float2 orig[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int X = (k & 1) ? icx : ix, Y = (k & 2) ? icy : iy, Z = (k & 4) ? icz : iz;
+        orig[k] = F[((int64_t)Z * N + Y) * N + X];
+    }
+    combine_octet(N, ix, iy, iz, orig, ph, wtab, cf, delta);
 }
 
 // Non-interlaced fcomb: only the window deconvolution and the 1/kF^3 normalisation of the
@@ -962,6 +971,86 @@ __global__ __launch_bounds__(256) void k_maps_rows(int C, int ld, int64_t nrows,
     }
 }
 
+// The last axis of the interlaced density's FFT fused with the combine.  rocFFT has transformed
+// every z plane of grid1 + i grid2 over (y, x) (G[z][y][x]); a work-group takes the octets
+// (ix, iy, iz) of XB consecutive ix, one iy and every iz: it loads the 4 XB z-columns (X, Y),
+// X in {ix, N - ix}, Y in {iy, N - iy}, that those octets read (runs of XB consecutive x: 128
+// bytes for XB = 16), transforms each over z in LDS (forward DFT of length N = A B: the inverse
+// four-step transform of the conjugate, conjugated again when read), then evaluates the octets
+// as k_combine_interlaced does.  In a column, z = j + B m sits at m (B + 1) + j and, after the two
+// steps, F[k1 + A k2] at k1 (B + 1) + k2: the second step's lanes (one k1 each) read B + 1
+// entries apart, on distinct LDS banks.  Threads: 4 XB x 8 z rows.
+template <int N, int A, int B, int XB>
+__global__ __launch_bounds__(32 * XB) void k_zfft_combine(const float2 *ph, const float *wtab,
+                                                          float cf, const float2 *tw,
+                                                          const float2 *G, float2 *delta) {
+    static_assert(A * B == N, "factors of N");
+    constexpr int NT = 32 * XB, NC = 4 * XB, PB = B + 1, CS = A * PB + 1, H = N / 2 + 1;
+    __shared__ float2 col[NC * CS];
+    const int x0 = blockIdx.x * XB, iy = blockIdx.y;
+    const int icy = (N - iy) % N;
+    // column c = xi + XB (side + 2 ysel): X = side ? N - ix : ix (mod N), Y = ysel ? icy : iy;
+    // ix past N / 2 (the last block) loads the cells of ix mod N and writes nothing
+    {
+        const int xi = threadIdx.x % XB, side = (threadIdx.x / XB) & 1;
+        const int ysel = (threadIdx.x / (2 * XB)) & 1, zr = threadIdx.x / (4 * XB);
+        const int ix = (x0 + xi) % N;
+        const int X = side ? (N - ix) % N : ix, Y = ysel ? icy : iy;
+        const int c = xi + XB * (side + 2 * ysel);
+        constexpr int ZS = NT / (4 * XB), NZ = N / ZS;
+        // all of a thread's loads issue before its LDS stores (NZ in flight per thread)
+        float2 v[NZ];
+#pragma unroll
+        for (int i = 0; i < NZ; ++i) v[i] = G[((int64_t)(zr + ZS * i) * N + Y) * N + X];
+#pragma unroll
+        for (int i = 0; i < NZ; ++i) {
+            const int z = zr + ZS * i;
+            col[c * CS + (z / B) * PB + z % B] = make_float2(v[i].x, -v[i].y);
+        }
+    }
+    __syncthreads();
+    // step 1: per column and j < B, the A entries z = j + B m over m, times W_N^{j k1}, in place
+    for (int t = threadIdx.x; t < NC * B; t += NT) {
+        const int c = t / B, j = t - c * B;
+        float2 *cc = col + c * CS + j;
+        float2 x[A];
+#pragma unroll
+        for (int m = 0; m < A; ++m) x[m] = cc[m * PB];
+        rdft<A>(x);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) cc[k1 * PB] = k1 ? cmul(x[k1], tw[j * k1]) : x[0];
+    }
+    __syncthreads();
+    // step 2: per column and k1 < A, the B entries k1 (B + 1) + j over j, in place
+    for (int t = threadIdx.x; t < NC * A; t += NT) {
+        const int c = t / A, k1 = t - c * A;
+        float2 *cc = col + c * CS + k1 * PB;
+        float2 x[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) x[j] = cc[j];
+        rdft<B>(x);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) cc[k2] = x[k2];
+    }
+    __syncthreads();
+    auto F = [&](int c, int Z) {     // forward transform of column c at Z
+        const float2 v = col[c * CS + (Z % A) * PB + Z / A];
+        return make_float2(v.x, -v.y);
+    };
+    for (int t = threadIdx.x; t < XB * H; t += NT) {
+        const int xi = t % XB, iz = t / XB, ix = x0 + xi;
+        if (ix > N / 2) continue;
+        const int icz = (N - iz) % N;
+        float2 orig[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = xi + XB * ((k & 1) + 2 * ((k >> 1) & 1));
+            orig[k] = F(c, (k & 4) ? icz : iz);
+        }
+        combine_octet(N, ix, iy, iz, orig, ph, wtab, cf, delta);
+    }
+}
+
 }  // namespace
 
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
@@ -1053,6 +1142,29 @@ void launch_combine(int N, int interlacing, const float *wtab, const float2 *ph,
         const int64_t n = (int64_t)N * N * h;
         hipLaunchKernelGGL(k_combine_single, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, N,
                            wtab, cf, G1, delta);
+    }
+    PBI4_HIP(hipGetLastError());
+}
+
+bool zfft_combine_supported(int N) { return N == 128 || N == 192 || N == 256 || N == 384; }
+
+void launch_zfft_combine(int N, const float *wtab, const float2 *ph, const float2 *tw, float cf,
+                         const float2 *G, float2 *delta, hipStream_t s) {
+    PBI4_CHECK(ph != nullptr && tw != nullptr, PBI4_EINVAL, "the fused z transform needs its tables");
+    auto run = [&](auto kern, int xb) {
+        const dim3 grid((unsigned)ceil_div(N / 2 + 1, xb), (unsigned)(N / 2 + 1));
+        hipLaunchKernelGGL(kern, grid, dim3(32 * xb), 0, s, ph, wtab, cf, tw, G, delta);
+    };
+    // 8 x-columns per side (64-byte runs, 66 KB of LDS at 256^3, two work-groups per CU): 16
+    // (128-byte runs, 140 KB, one work-group per CU) took 78.5 against 90.5 us alone at 256^3 but
+    // the six-slot serving stream ran slower with it (435 against 451 catalogues/s), its LDS
+    // keeping other slots' work-groups off the CU
+    switch (N) {
+        case 128: run(k_zfft_combine<128, 16, 8, 8>, 8); break;
+        case 192: run(k_zfft_combine<192, 16, 12, 8>, 8); break;
+        case 256: run(k_zfft_combine<256, 16, 16, 8>, 8); break;
+        case 384: run(k_zfft_combine<384, 16, 24, 8>, 8); break;
+        default: fail(PBI4_EUNSUPPORTED, "no fused z transform for this grid");
     }
     PBI4_HIP(hipGetLastError());
 }
