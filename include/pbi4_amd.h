@@ -87,6 +87,18 @@ size_t pbi4_density_workspace(pbi4_ctx *ctx, const pbi4_density_desc *d);
 int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float *grids,
                          void *delta, void *ws, size_t ws_bytes, void *stream);
 
+/* pbi4_assign followed by pbi4_compute_density in one call: the two steps the reference runs
+ * back to back (pbi4/pbi4.py:304-341, DENSITIES.assign_* then compute_density).  On interlaced
+ * grids 128, 192, 256 and 384 the gather of the tile regions is fused with the FFT's x axis and
+ * the FFT runs in grids (no rocFFT pass); elsewhere, or with PBI4_DENSITY_FFT set, it is exactly
+ * pbi4_assign + pbi4_compute_density.  grids (2 N^3 floats, or N^3 single grid) is scratch here
+ * and holds no defined values afterwards; ws must hold max(pbi4_assign_workspace,
+ * pbi4_density_workspace) bytes.  ad and dd must name the same grid, order and interlacing.
+ */
+int pbi4_assign_density(pbi4_ctx *ctx, const pbi4_assign_desc *ad, const pbi4_density_desc *dd,
+                        const float *pos, const float *vel, int64_t npart, float *grids,
+                        void *delta, void *ws, size_t ws_bytes, void *stream);
+
 /* Full-array forms of the reference routines for callers that hold complete N^3 arrays (the
  * f2py-named Python functions).  Arrays are complex float2 [z][y][x] (Fortran (x,y,z) order).
  * pbi4_fcomb: powerI4.fcomb (powerI4.f90:251-378) on fdcl = FFT(grid1 + i grid2), all N^3

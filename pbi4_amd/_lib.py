@@ -63,6 +63,8 @@ SIGNATURES = {
     "pbi4_density_workspace": (c_size_t, [c_void_p, P(DensityDesc)]),
     "pbi4_compute_density": (c_int, [c_void_p, P(DensityDesc), c_void_p, c_void_p, c_void_p,
                                      c_size_t, c_void_p]),
+    "pbi4_assign_density": (c_int, [c_void_p, P(AssignDesc), P(DensityDesc), c_void_p, c_void_p,
+                                    c_int64, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pbi4_fcomb": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p, c_void_p, c_void_p]),
     "pbi4_compute_qk": (c_int, [c_void_p, c_int, c_int, P(c_int), c_void_p, c_void_p, c_void_p]),
     "pbi4_catalogue_positions": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int64, c_int,

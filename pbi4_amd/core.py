@@ -135,6 +135,39 @@ class Context:
                                                _ptr(ws), ws.numel(), _stream(stream)))
         return out
 
+    def assign_density(self, pos, grid, order, interlacing, box, npart, vel=None, rsd=None,
+                       normalised=True, grids=None, out=None, ws=None, stream=None):
+        """assign followed by density in one call (pbi4_assign_density): the half spectrum
+        (N, N, N/2+1) complex64.  grids: optional (ng, N, N, N) float32 scratch (its contents
+        are undefined afterwards); ws: optional uint8 workspace of at least
+        max(assign_workspace, density_workspace) bytes."""
+        _check_dev(pos, "pos", torch.float32)
+        if pos.dim() != 2 or pos.shape[0] != 3:
+            raise Pbi4Error("pos must have shape (3, npart)")
+        n = pos.shape[1]
+        if vel is not None:
+            _check_dev(vel, "vel", torch.float32)
+            if vel.shape != pos.shape:
+                raise Pbi4Error("vel must have the shape of pos")
+        ad = AssignDesc(int(grid), int(order), int(bool(interlacing)),
+                        _lib.POS_NORMALISED if normalised else _lib.POS_BOX, float(box),
+                        (ctypes.c_float * 3)(*(rsd if rsd is not None else (0.0, 0.0, 0.0))))
+        dd = DensityDesc(int(grid), int(order), int(bool(interlacing)), float(box), float(npart))
+        ng = 2 if interlacing else 1
+        if grids is None:
+            grids = torch.empty((ng, grid, grid, grid), dtype=torch.float32, device=self.device)
+        _check_dev(grids, "grids", torch.float32)
+        if grids.numel() < ng * grid ** 3:
+            raise Pbi4Error("grids holds fewer than ng * N^3 floats")
+        if out is None:
+            out = torch.empty((grid, grid, grid // 2 + 1), dtype=torch.complex64, device=self.device)
+        need = max(_assign_ws_bytes(ad, n), lib().pbi4_density_workspace(self.h, ctypes.byref(dd)))
+        ws = self._ws_or(ws, need, "ws")
+        self._check(lib().pbi4_assign_density(self.h, ctypes.byref(ad), ctypes.byref(dd), _ptr(pos),
+                                              _ptr(vel), n, _ptr(grids), _ptr(out), _ptr(ws),
+                                              ws.numel(), _stream(stream)))
+        return out
+
     # ------------------------------------------------------------- full-array forms
     def fcomb_full(self, fdcl, grid, nintp, interlacing, box, out=None, stream=None):
         """powerI4.fcomb on a full (N, N, N) complex64 [z][y][x] array."""

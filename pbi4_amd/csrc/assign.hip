@@ -18,6 +18,7 @@
 //        single grid: h = u - nint(u)                          (:114-124)
 //   NGP  cell nint(u)                                          (:225-233)
 #include "common.h"
+#include "dft.h"
 
 #include <type_traits>
 
@@ -994,6 +995,156 @@ void k_gather(int N, int T, int NT, int R, int ngrids, const float *regions, flo
         if (k < T) out[(int64_t)k * N * N] = s[k];
 }
 
+// The interlaced gather fused with the first axis of the density's FFT: row y of plane z of
+// grid1 + i grid2, each cell summed from its region entries in k_gather's order (so the cells
+// are bitwise k_gather's), then transformed along x in LDS: out[z][y][k] = sum_x (grid1 +
+// i grid2)[z][y][x] e^{-2 pi i x k / N}, the forward DFT of length N = A B taken as the
+// inverse four-step transform of the conjugate, conjugated again when written.  In a row,
+// x = j + B m sits at m (B + 1) + j and, after the two steps, F[k1 + A k2] at k1 (B + 1) + k2.
+// One work-group per (16 rows, plane z); thread t sums the cells t + 256 i (a wave's 64 cells
+// share one row, so the y options and the z neighbour are wave-uniform branches), kCellBatch
+// at a time with their loads in flight; the x option of a cell away from its tile's x edges
+// re-reads the home entry and is not added (one load instruction either way).  The tile edge
+// T is a template parameter: the region's piece offsets fold to a few multiply-adds.
+// tw[k] = e^{2 pi i k / N}.
+constexpr int kXfftRows = 16;
+constexpr int kCellBatch = 4;
+
+template <int N, int A, int B, int T>
+__global__ __launch_bounds__(256) void k_gather_xfft(const float *regions, const float2 *tw,
+                                                     float2 *out) {
+    static_assert(A * B == N && N % 64 == 0 && N % T == 0, "factors of N");
+    constexpr int NT = N / T, R = T + 2 * kHalo, R3 = R * R * R;
+    constexpr int PB = B + 1, RS = A * PB + 1, CPT = N * kXfftRows / 256;
+    static_assert(CPT % kCellBatch == 0, "cells per thread");
+    static_assert((int64_t)NT * NT * NT * 2 * R3 < (int64_t)1 << 31, "32-bit region offsets");
+    __shared__ float2 rows[kXfftRows * RS];
+    const int y0 = blockIdx.x * kXfftRows, z = blockIdx.y;
+    const int tz = z / T, kz = z - tz * T;
+    // the z-neighbour entry of this plane: cells kz < 2 add the previous z tile's entry at
+    // kz + T + 2, cells kz >= T - 2 the next tile's at kz - T + 2
+    const bool zn = kz < kHalo || kz >= T - kHalo;
+    const int tzn = kz < kHalo ? (tz + NT - 1) % NT : (tz + 1) % NT;
+    const int czh = kz + kHalo, czn = kz < kHalo ? kz + T + kHalo : kz - T + kHalo;
+    const int zb = tz * NT * NT, znb = tzn * NT * NT;
+    // region offset of cell (a, b, c) of tile (tzb + ty NT + tx), grid 0 (grid 1 is R3 further)
+    auto at = [&](int tzb, int ty, int tx, int a, int b, int c) {
+        return (uint32_t)((tzb + ty * NT + tx) * 2 * R3 + region_index(a, b, c, T, R));
+    };
+    // t = home-z entry (+ z-neighbour entry) of option (tile x, a; tile y, b), both grids
+    auto load2 = [&](auto has_zn, int tx, int a, int ty, int b, float (&v)[2], float (&w)[2]) {
+        const float *r = regions + at(zb, ty, tx, a, b, czh);
+        v[0] = r[0];
+        v[1] = r[R3];
+        if constexpr (decltype(has_zn)::value) {
+            const float *rn = regions + at(znb, ty, tx, a, b, czn);
+            w[0] = rn[0];
+            w[1] = rn[R3];
+        }
+    };
+    auto sum_batch = [&](auto has_zn, int ib) {
+        constexpr bool ZN = decltype(has_zn)::value;
+        float s0[kCellBatch], s1[kCellBatch];
+        int tx[kCellBatch], a0[kCellBatch], ntx[kCellBatch], a1[kCellBatch], xe[kCellBatch];
+        int ty[kCellBatch], b0[kCellBatch], nty[kCellBatch], b1[kCellBatch], ye[kCellBatch];
+#pragma unroll
+        for (int c = 0; c < kCellBatch; ++c) {
+            const int e = threadIdx.x + 256 * (ib + c);
+            const int x = e % N, y = y0 + e / N;
+            tx[c] = x / T;
+            ty[c] = y / T;
+            const int cx = x - tx[c] * T, cy = y - ty[c] * T;
+            a0[c] = a1[c] = cx + kHalo;
+            b0[c] = b1[c] = cy + kHalo;
+            ntx[c] = tx[c];
+            nty[c] = ty[c];
+            xe[c] = ye[c] = 0;
+            if (cx < kHalo) { ntx[c] = (tx[c] + NT - 1) % NT; a1[c] = cx + T + kHalo; xe[c] = 1; }
+            else if (cx >= T - kHalo) { ntx[c] = (tx[c] + 1) % NT; a1[c] = cx - T + kHalo; xe[c] = 1; }
+            if (cy < kHalo) { nty[c] = (ty[c] + NT - 1) % NT; b1[c] = cy + T + kHalo; ye[c] = 1; }
+            else if (cy >= T - kHalo) { nty[c] = (ty[c] + 1) % NT; b1[c] = cy - T + kHalo; ye[c] = 1; }
+        }
+        // options (0, 0) and (1, 0), every cell: s = t00 (+ t10 at an x edge)
+        {
+            float v[kCellBatch][2][2], w[kCellBatch][2][2];
+#pragma unroll
+            for (int c = 0; c < kCellBatch; ++c) {
+                load2(has_zn, tx[c], a0[c], ty[c], b0[c], v[c][0], w[c][0]);
+                load2(has_zn, ntx[c], a1[c], ty[c], b0[c], v[c][1], w[c][1]);
+            }
+#pragma unroll
+            for (int c = 0; c < kCellBatch; ++c) {
+                float t[2][2];
+#pragma unroll
+                for (int o = 0; o < 2; ++o)
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) t[o][g] = ZN ? v[c][o][g] + w[c][o][g] : v[c][o][g];
+                s0[c] = xe[c] ? t[0][0] + t[1][0] : t[0][0];
+                s1[c] = xe[c] ? t[0][1] + t[1][1] : t[0][1];
+            }
+        }
+        // options (0, 1) and (1, 1), rows at a y edge (whole waves): s += t01 (+ t11)
+#pragma unroll
+        for (int c = 0; c < kCellBatch; ++c) {
+            if (!ye[c]) continue;
+            float v[2][2], w[2][2];
+            load2(has_zn, tx[c], a0[c], nty[c], b1[c], v[0], w[0]);
+            load2(has_zn, ntx[c], a1[c], nty[c], b1[c], v[1], w[1]);
+            float t[2][2];
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int g = 0; g < 2; ++g) t[o][g] = ZN ? v[o][g] + w[o][g] : v[o][g];
+            s0[c] += t[0][0];
+            s1[c] += t[0][1];
+            if (xe[c]) { s0[c] += t[1][0]; s1[c] += t[1][1]; }
+        }
+#pragma unroll
+        for (int c = 0; c < kCellBatch; ++c) {
+            const int e = threadIdx.x + 256 * (ib + c);
+            const int x = e % N, yr = e / N;
+            rows[yr * RS + (x / B) * PB + x % B] = make_float2(s0[c], -s1[c]);
+        }
+    };
+    for (int ib = 0; ib < CPT; ib += kCellBatch) {
+        if (zn) sum_batch(std::true_type{}, ib);
+        else sum_batch(std::false_type{}, ib);
+    }
+    __syncthreads();
+    // step 1: per row and j < B, the A entries x = j + B m over m, times W_N^{j k1}, in place
+    for (int t = threadIdx.x; t < kXfftRows * B; t += 256) {
+        const int r = t / B, j = t - r * B;
+        float2 *rr = rows + r * RS + j;
+        float2 a[A];
+#pragma unroll
+        for (int m = 0; m < A; ++m) a[m] = rr[m * PB];
+        rdft<A>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) rr[k1 * PB] = k1 ? cmul(a[k1], tw[j * k1]) : a[0];
+    }
+    __syncthreads();
+    // step 2: per row and k1 < A, the B entries k1 (B + 1) + j over j, in place
+    for (int t = threadIdx.x; t < kXfftRows * A; t += 256) {
+        const int r = t / A, k1 = t - r * A;
+        float2 *rr = rows + r * RS + k1 * PB;
+        float2 a[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) a[j] = rr[j];
+        rdft<B>(a);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) rr[k2] = a[k2];
+    }
+    __syncthreads();
+    float2 *o = out + ((int64_t)z * N + y0) * N;
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int e = threadIdx.x + 256 * i;
+        const int k = e % N, yr = e / N;
+        const float2 f = rows[yr * RS + (k % A) * PB + k / A];
+        o[e] = make_float2(f.x, -f.y);
+    }
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1063,8 +1214,8 @@ size_t assign_workspace(const AssignPlan &p, int64_t npart) {
     return a.used + 256;
 }
 
-void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t npart,
-                float *grids, Arena &ws, hipStream_t s, KernelTimer *timer) {
+const float *assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t npart,
+                       float *grids, Arena &ws, hipStream_t s, KernelTimer *timer) {
     PBI4_CHECK(npart >= 0 && npart < (int64_t)0xFFFFFFFFll, PBI4_EINVAL,
                "npart must be in [0, 2^32)");
     PBI4_CHECK(pos != nullptr || npart == 0, PBI4_EINVAL, "pos is NULL");
@@ -1134,11 +1285,41 @@ void assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t
         }
 #undef PBI4_DEPOSIT
     }
-    {
+    if (grids) {
         TimedScope ts(timer, "pbi4_gather", s);
         hipLaunchKernelGGL(k_gather, dim3((unsigned)p.ntiles, p.ngrids), dim3(256), 0, s, p.N, p.T,
                            p.NT, p.R, p.ngrids, regions, grids);
     }
+    PBI4_HIP(hipGetLastError());
+    return regions;
+}
+
+bool gather_xfft_supported(const AssignPlan &p) {
+    if (p.ngrids != 2 || p.R != p.T + 2 * kHalo) return false;
+    switch (p.N) {
+        case 128: case 256: return p.T == 8 || p.T == 16;
+        case 192: case 384: return p.T == 8 || p.T == 12 || p.T == 16;
+        default: return false;
+    }
+}
+
+void launch_gather_xfft(const AssignPlan &p, const float *regions, const float2 *tw, float2 *out,
+                        hipStream_t s) {
+    PBI4_CHECK(gather_xfft_supported(p), PBI4_EUNSUPPORTED,
+               "fused gather + x transform: interlaced grids 128, 192, 256, 384 with tile edge 8, "
+               "12 or 16 only");
+    const dim3 grid(p.N / kXfftRows, p.N);
+    auto run = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, regions, tw, out); };
+#define PBI4_GX(N, A, B) \
+    (p.T == 8 ? run(k_gather_xfft<N, A, B, 8>) : p.T == 16 ? run(k_gather_xfft<N, A, B, 16>) \
+              : run(k_gather_xfft<N, A, B, (N % 12 == 0 ? 12 : 16)>))
+    switch (p.N) {
+        case 128: PBI4_GX(128, 16, 8); break;
+        case 192: PBI4_GX(192, 16, 12); break;
+        case 256: PBI4_GX(256, 16, 16); break;
+        default: PBI4_GX(384, 16, 24); break;
+    }
+#undef PBI4_GX
     PBI4_HIP(hipGetLastError());
 }
 

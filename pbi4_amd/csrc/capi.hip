@@ -969,6 +969,47 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
     });
 }
 
+int pbi4_assign_density(pbi4_ctx *ctx, const pbi4_assign_desc *ad, const pbi4_density_desc *dd,
+                        const float *pos, const float *vel, int64_t npart, float *grids,
+                        void *delta, void *ws, size_t ws_bytes, void *stream) {
+    bool fused = false;
+    int rc = guarded(ctx, [&] {
+        PBI4_CHECK(ad && dd && grids && delta && ws, PBI4_EINVAL, "NULL argument");
+        PBI4_CHECK(npart >= 0 && (npart == 0 || pos), PBI4_EINVAL,
+                   "positions missing or a negative particle count");
+        check_density(*dd);
+        PBI4_CHECK(ad->grid == dd->grid && ad->order == dd->order &&
+                       ad->interlacing == dd->interlacing,
+                   PBI4_EINVAL, "assignment and density descriptors differ in grid/order/interlacing");
+        AssignPlan p = make_assign_plan(*ad, npart);
+        fused = gather_xfft_supported(p) && !std::getenv("PBI4_DENSITY_FFT");
+        if (!fused) return;
+        hipStream_t s = (hipStream_t)stream;
+        Arena a(ws, ws_bytes);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        const float *regions = assign_run(p, pos, vel, npart, nullptr, a, s, &ctx->timer);
+        const int N = dd->grid;
+        const float2 *tw = get_twiddle(ctx, N);
+        float2 *G = reinterpret_cast<float2 *>(grids);
+        {
+            TimedScope ts(&ctx->timer, "pbi4_gather", s);
+            launch_gather_xfft(p, regions, tw, G, s);
+        }
+        {
+            TimedScope ts(&ctx->timer, "pbi4_r2c", s);
+            launch_yfft(N, tw, G, s);
+        }
+        const double cf = 1.0 / (4.0 * kf3_fcomb(dd->box)) / dd->npart;
+        TimedScope ts(&ctx->timer, "pbi4_combine", s);
+        launch_zfft_combine(N, get_window(ctx, N, dd->order), get_phase(ctx, N), tw, (float)cf, G,
+                            static_cast<float2 *>(delta), s);
+    });
+    if (rc != PBI4_OK || fused) return rc;
+    rc = pbi4_assign(ctx, ad, pos, vel, npart, grids, ws, ws_bytes, stream);
+    if (rc != PBI4_OK) return rc;
+    return pbi4_compute_density(ctx, dd, grids, delta, ws, ws_bytes, stream);
+}
+
 int pbi4_fcomb(pbi4_ctx *ctx, int grid, int nintp, int interlacing, float box, const void *fdcl,
                void *dcl, void *stream) {
     return guarded(ctx, [&] {

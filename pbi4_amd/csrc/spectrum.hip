@@ -1,16 +1,10 @@
 // Fourier-space kernels: interlacing combine (fcomb), P_l / shell-power binning, shell filter.
 #include "common.h"
+#include "dft.h"
 
 namespace pbi4 {
 
 namespace {
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
 // ------------------------------------------------------------------ fcomb (powerI4.f90:251-378)
 
@@ -648,116 +642,6 @@ __global__ void k_qk(int N, int norder, int4 idx, const float2 *F, float2 *out) 
 
 // ------------------------------------------------------------------ maps: 2D c2r through LDS
 
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 times_i(float2 a) { return make_float2(-a.y, a.x); }
-
-// In-place inverse DFT of R values, y_k = sum_m a_m e^{+2 pi i m k / R}.  Constants are cos and sin
-// of 2 pi / 3, 2 pi / 5 and 4 pi / 5 rounded once from their double values.
-template <int R>
-__device__ __forceinline__ void idft(float2 *a) {
-    if constexpr (R == 2) {
-        const float2 t = a[0];
-        a[0] = cadd(t, a[1]);
-        a[1] = csub(t, a[1]);
-    } else if constexpr (R == 3) {
-        const float2 s = cadd(a[1], a[2]);
-        const float2 u = times_i(cscale(csub(a[1], a[2]), 0.8660254037844386f));
-        const float2 t = make_float2(a[0].x - 0.5f * s.x, a[0].y - 0.5f * s.y);
-        a[0] = cadd(a[0], s);
-        a[1] = cadd(t, u);
-        a[2] = csub(t, u);
-    } else if constexpr (R == 4) {
-        const float2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
-        const float2 s13 = cadd(a[1], a[3]), d13 = times_i(csub(a[1], a[3]));
-        a[0] = cadd(s02, s13);
-        a[2] = csub(s02, s13);
-        a[1] = cadd(d02, d13);
-        a[3] = csub(d02, d13);
-    } else {
-        const float c1 = 0.30901699437494745f, c2 = -0.8090169943749475f;
-        const float s1 = 0.9510565162951535f, s2 = 0.5877852522924731f;
-        const float2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
-        const float2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
-        const float2 t1 = make_float2(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
-        const float2 t2 = make_float2(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
-        const float2 u1 = times_i(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
-        const float2 u2 = times_i(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
-        a[0] = cadd(a[0], cadd(b1, b2));
-        a[1] = cadd(t1, u1);
-        a[4] = csub(t1, u1);
-        a[2] = cadd(t2, u2);
-        a[3] = csub(t2, u2);
-    }
-}
-
-// Compile-time inverse DFT of L values in registers, natural order in and out:
-// X[k0 + M k1] = sum_m W_L^{m k0} W_r^{m k1} Y_m[k0] with L = r M, Y_m the length-M transform of
-// x[m + r i] (decimation in time); radices 5, 4, 3, 2 first to last.  The twiddles W_L^e =
-// e^{2 pi i e / L} are compile-time constants from a Taylor series in double, rounded once.
-constexpr double kPiD = 3.14159265358979323846;
-constexpr double cx_sin(double x) {
-    double term = x, sum = x;
-    for (int n = 1; n < 30; ++n) {
-        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
-        sum += term;
-    }
-    return sum;
-}
-constexpr double cx_cos(double x) {
-    double term = 1.0, sum = 1.0;
-    for (int n = 1; n < 30; ++n) {
-        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
-        sum += term;
-    }
-    return sum;
-}
-template <int L>
-struct TwTab {
-    float re[L], im[L];
-    constexpr TwTab() : re(), im() {
-        for (int e = 0; e < L; ++e) {
-            const int f = 2 * e > L ? e - L : e;            // angle in [-pi, pi]
-            const double a = 2.0 * kPiD * (double)f / (double)L;
-            re[e] = (float)cx_cos(a);
-            im[e] = (float)cx_sin(a);
-        }
-    }
-};
-template <int L>
-__device__ __forceinline__ float2 tw_const(int e) {
-    constexpr TwTab<L> t{};
-    return make_float2(t.re[e % L], t.im[e % L]);
-}
-
-template <int L>
-__device__ __forceinline__ void rdft(float2 *x) {
-    if constexpr (L == 1) {
-        return;
-    } else if constexpr (L == 2 || L == 3 || L == 4 || L == 5) {
-        idft<L>(x);
-    } else {
-        constexpr int r = L % 5 == 0 ? 5 : L % 4 == 0 ? 4 : L % 3 == 0 ? 3 : 2;
-        constexpr int M = L / r;
-        static_assert(r * M == L, "lengths are 2,3,5-smooth");
-        float2 sub[r][M];
-#pragma unroll
-        for (int m = 0; m < r; ++m) {
-#pragma unroll
-            for (int i = 0; i < M; ++i) sub[m][i] = x[m + r * i];
-            rdft<M>(sub[m]);
-        }
-#pragma unroll
-        for (int k0 = 0; k0 < M; ++k0) {
-            float2 a[r];
-#pragma unroll
-            for (int m = 0; m < r; ++m) a[m] = m * k0 ? cmul(sub[m][k0], tw_const<L>(m * k0)) : sub[m][k0];
-            idft<r>(a);
-#pragma unroll
-            for (int k1 = 0; k1 < r; ++k1) x[k0 + M * k1] = a[k1];
-        }
-    }
-}
-
 // One work-group per plane (map m, z); four-step transforms with compile-time factors.
 // Columns (length N' = A1 B1, only x' < C): thread (c, j < B1) loads rows q = j + B1 m, m < A1,
 // transforms them over m, multiplies by W_N'^{j k1} and stores at k1 B1 + j; thread (c, k1 < A1)
@@ -1051,7 +935,74 @@ __global__ __launch_bounds__(32 * XB) void k_zfft_combine(const float2 *ph, cons
     }
 }
 
+// The middle axis of the interlaced density's FFT, in place: a work-group loads the 16 y-columns
+// (z, x0 .. x0 + 15) of G[z][y][x] (128-byte runs), transforms each over y in LDS (forward DFT of
+// length N = A B as the inverse four-step transform of the conjugate; same layout as
+// k_zfft_combine) and writes them back.  Threads: 16 x by 16 y rows.
+template <int N, int A, int B>
+__global__ __launch_bounds__(256) void k_yfft(const float2 *tw, float2 *G) {
+    static_assert(A * B == N && N % 16 == 0, "factors of N");
+    constexpr int XB = 16, PB = B + 1, CS = A * PB + 1, NY = N / 16;
+    __shared__ float2 col[XB * CS];
+    const int x0 = blockIdx.x * XB, z = blockIdx.y;
+    const int xi = threadIdx.x % XB, yr = threadIdx.x / XB;
+    float2 *g = G + (int64_t)z * N * N + x0 + xi;
+    {
+        float2 v[NY];
+#pragma unroll
+        for (int i = 0; i < NY; ++i) v[i] = g[(int64_t)(yr + 16 * i) * N];
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int y = yr + 16 * i;
+            col[xi * CS + (y / B) * PB + y % B] = make_float2(v[i].x, -v[i].y);
+        }
+    }
+    __syncthreads();
+    // step 1: per column and j < B, the A entries y = j + B m over m, times W_N^{j k1}, in place
+    for (int t = threadIdx.x; t < XB * B; t += 256) {
+        const int c = t / B, j = t - c * B;
+        float2 *cc = col + c * CS + j;
+        float2 x[A];
+#pragma unroll
+        for (int m = 0; m < A; ++m) x[m] = cc[m * PB];
+        rdft<A>(x);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) cc[k1 * PB] = k1 ? cmul(x[k1], tw[j * k1]) : x[0];
+    }
+    __syncthreads();
+    // step 2: per column and k1 < A, the B entries k1 (B + 1) + j over j, in place
+    for (int t = threadIdx.x; t < XB * A; t += 256) {
+        const int c = t / A, k1 = t - c * A;
+        float2 *cc = col + c * CS + k1 * PB;
+        float2 x[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) x[j] = cc[j];
+        rdft<B>(x);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) cc[k2] = x[k2];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NY; ++i) {
+        const int y = yr + 16 * i;
+        const float2 f = col[xi * CS + (y % A) * PB + y / A];
+        g[(int64_t)y * N] = make_float2(f.x, -f.y);
+    }
+}
+
 }  // namespace
+
+void launch_yfft(int N, const float2 *tw, float2 *G, hipStream_t s) {
+    PBI4_CHECK(zfft_combine_supported(N), PBI4_EUNSUPPORTED, "y transform: grids 128, 192, 256, 384");
+    const dim3 grid(N / 16, N);
+    switch (N) {
+        case 128: hipLaunchKernelGGL((k_yfft<128, 16, 8>), grid, dim3(256), 0, s, tw, G); break;
+        case 192: hipLaunchKernelGGL((k_yfft<192, 16, 12>), grid, dim3(256), 0, s, tw, G); break;
+        case 256: hipLaunchKernelGGL((k_yfft<256, 16, 16>), grid, dim3(256), 0, s, tw, G); break;
+        default: hipLaunchKernelGGL((k_yfft<384, 16, 24>), grid, dim3(256), 0, s, tw, G); break;
+    }
+    PBI4_HIP(hipGetLastError());
+}
 
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,
                        float2 *out, hipStream_t s) {

@@ -137,12 +137,9 @@ class CataloguePlan:
         with torch.cuda.stream(s):
             for f in m.fields:
                 rsd = tuple(float(np.float32(self.rsd_coef * l)) for l in f.los) if f.rsd else None
-                g = ctx.assign(pos, N, m.order, m.interlacing, vel=vel if f.rsd else None,
-                               box=m.box, rsd=rsd, normalised=False, out=grids, ws=scratch,
-                               stream=s)
-                d = ctx.density(g, N, m.order, m.interlacing, m.box, npart, out=delta,
-                                ws=scratch, stream=s)
-                del g
+                d = ctx.assign_density(pos, N, m.order, m.interlacing, m.box, npart,
+                                       vel=vel if f.rsd else None, rsd=rsd, normalised=False,
+                                       grids=grids, out=delta, ws=scratch, stream=s)
                 desc = self.desc[f]
                 desc.npart = int(npart)
                 if precision is not None:
