@@ -355,6 +355,15 @@ def main():
             torch.cuda.synchronize()
         out["stage_ms_rsd_field"] = {nm: round(ev[i].elapsed_time(ev[i + 1]), 4)
                                      for i, nm in enumerate(names)}
+        # the serving pipeline's route: assignment and density in one call (fused gather and
+        # x transform, y pass, z transform + combine)
+        for _ in range(2):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(s)
+            ctx.assign_density(p, N, 4, True, BOX, npart, vel=v, rsd=rsd, normalised=False)
+            ev[1].record(s)
+            torch.cuda.synchronize()
+        out["stage_ms_rsd_field"]["assign_density_one_call"] = round(ev[0].elapsed_time(ev[1]), 4)
         log(f"redshift-space field stages (ms): {out['stage_ms_rsd_field']}")
 
     # ---------------------------------------------------------------- speedup ladder

@@ -1001,22 +1001,28 @@ void k_gather(int N, int T, int NT, int R, int ngrids, const float *regions, flo
 // i grid2)[z][y][x] e^{-2 pi i x k / N}, the forward DFT of length N = A B taken as the
 // inverse four-step transform of the conjugate, conjugated again when written.  In a row,
 // x = j + B m sits at m (B + 1) + j and, after the two steps, F[k1 + A k2] at k1 (B + 1) + k2.
-// One work-group per (16 rows, plane z); thread t sums the cells t + 256 i (a wave's 64 cells
-// share one row, so the y options and the z neighbour are wave-uniform branches), kCellBatch
-// at a time with their loads in flight; the x option of a cell away from its tile's x edges
-// re-reads the home entry and is not added (one load instruction either way).  The tile edge
-// T is a template parameter: the region's piece offsets fold to a few multiply-adds.
-// tw[k] = e^{2 pi i k / N}.
-constexpr int kXfftRows = 16;
-constexpr int kCellBatch = 4;
+// One work-group per (8 rows, plane z; 16 rows on 192, so that each thread has whole quads).
+// Thread t sums the quads (4 cells along x) t + 256 i:
+// along x a tile's home cells are contiguous in every piece, so a quad's home entries are one
+// 16-byte load per grid and option (piece offsets, row lengths T and quad starts are multiples
+// of 4 floats), and a quad at a tile's x edge reads its 2 neighbour entries as one 8-byte load
+// (any other quad issues the same load on its own tile's entries T + 2, T + 3 and drops them;
+// pointing it at the quad's own home entries instead, the line the home load fetches, measured
+// 119 against 106 us at 256^3).
+// The y options are taken only on rows at a tile's y edge and the z neighbour only on planes at
+// a tile's z edge (whole waves or the whole work-group).  The tile edge T (8, 12 or 16) is a
+// template parameter.  tw[k] = e^{2 pi i k / N}.
+constexpr int xfft_rows(int N) { return N / 4 * 8 % 256 == 0 ? 8 : 16; }
 
 template <int N, int A, int B, int T>
 __global__ __launch_bounds__(256) void k_gather_xfft(const float *regions, const float2 *tw,
                                                      float2 *out) {
-    static_assert(A * B == N && N % 64 == 0 && N % T == 0, "factors of N");
+    static_assert(A * B == N && N % 64 == 0 && N % T == 0 && T % 4 == 0 && T >= 8, "shapes");
     constexpr int NT = N / T, R = T + 2 * kHalo, R3 = R * R * R;
-    constexpr int PB = B + 1, RS = A * PB + 1, CPT = N * kXfftRows / 256;
-    static_assert(CPT % kCellBatch == 0, "cells per thread");
+    constexpr int kXfftRows = xfft_rows(N);
+    constexpr int PB = B + 1, RS = A * PB + 1, QPR = N / 4, QPT = QPR * kXfftRows / 256;
+    constexpr int QB = QPT % 2 ? QPT : 2;        // quads per batch (all loads in flight)
+    static_assert(QPR * kXfftRows % 256 == 0 && QPT % QB == 0, "quads per thread");
     static_assert((int64_t)NT * NT * NT * 2 * R3 < (int64_t)1 << 31, "32-bit region offsets");
     __shared__ float2 rows[kXfftRows * RS];
     const int y0 = blockIdx.x * kXfftRows, z = blockIdx.y;
@@ -1027,86 +1033,98 @@ __global__ __launch_bounds__(256) void k_gather_xfft(const float *regions, const
     const int tzn = kz < kHalo ? (tz + NT - 1) % NT : (tz + 1) % NT;
     const int czh = kz + kHalo, czn = kz < kHalo ? kz + T + kHalo : kz - T + kHalo;
     const int zb = tz * NT * NT, znb = tzn * NT * NT;
-    // region offset of cell (a, b, c) of tile (tzb + ty NT + tx), grid 0 (grid 1 is R3 further)
+    // region offset of entry (a, b, c) of tile (tzb + ty NT + tx), grid 0 (grid 1 is R3 further)
     auto at = [&](int tzb, int ty, int tx, int a, int b, int c) {
-        return (uint32_t)((tzb + ty * NT + tx) * 2 * R3 + region_index(a, b, c, T, R));
+        return (tzb + ty * NT + tx) * 2 * R3 + region_index(a, b, c, T, R);
     };
-    // t = home-z entry (+ z-neighbour entry) of option (tile x, a; tile y, b), both grids
-    auto load2 = [&](auto has_zn, int tx, int a, int ty, int b, float (&v)[2], float (&w)[2]) {
-        const float *r = regions + at(zb, ty, tx, a, b, czh);
-        v[0] = r[0];
-        v[1] = r[R3];
-        if constexpr (decltype(has_zn)::value) {
-            const float *rn = regions + at(znb, ty, tx, a, b, czn);
-            w[0] = rn[0];
-            w[1] = rn[R3];
-        }
+    auto ld4 = [&](int off, float4 (&v)[2]) {
+        v[0] = *reinterpret_cast<const float4 *>(regions + off);
+        v[1] = *reinterpret_cast<const float4 *>(regions + off + R3);
     };
+    auto ld2 = [&](int off, float2 (&v)[2]) {
+        v[0] = *reinterpret_cast<const float2 *>(regions + off);
+        v[1] = *reinterpret_cast<const float2 *>(regions + off + R3);
+    };
+    auto add4 = [](float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); };
+    auto add2 = [](float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); };
+    // s[g][j] of the quad's cells j: s = t00 (+ t10 on the 2 cells at an x edge) (+ t01 (+ t11)),
+    // t = home-z entry (+ z-neighbour entry), k_gather's order
     auto sum_batch = [&](auto has_zn, int ib) {
         constexpr bool ZN = decltype(has_zn)::value;
-        float s0[kCellBatch], s1[kCellBatch];
-        int tx[kCellBatch], a0[kCellBatch], ntx[kCellBatch], a1[kCellBatch], xe[kCellBatch];
-        int ty[kCellBatch], b0[kCellBatch], nty[kCellBatch], b1[kCellBatch], ye[kCellBatch];
+        int tx[QB], a0[QB], ntx[QB], a1[QB], xe[QB];   // xe: 0 none, 1 cells 0-1, 2 cells 2-3
+        int ty[QB], b0[QB], nty[QB], b1[QB], ye[QB];
+        float4 s[QB][2];
 #pragma unroll
-        for (int c = 0; c < kCellBatch; ++c) {
-            const int e = threadIdx.x + 256 * (ib + c);
-            const int x = e % N, y = y0 + e / N;
-            tx[c] = x / T;
-            ty[c] = y / T;
-            const int cx = x - tx[c] * T, cy = y - ty[c] * T;
-            a0[c] = a1[c] = cx + kHalo;
-            b0[c] = b1[c] = cy + kHalo;
-            ntx[c] = tx[c];
-            nty[c] = ty[c];
-            xe[c] = ye[c] = 0;
-            if (cx < kHalo) { ntx[c] = (tx[c] + NT - 1) % NT; a1[c] = cx + T + kHalo; xe[c] = 1; }
-            else if (cx >= T - kHalo) { ntx[c] = (tx[c] + 1) % NT; a1[c] = cx - T + kHalo; xe[c] = 1; }
-            if (cy < kHalo) { nty[c] = (ty[c] + NT - 1) % NT; b1[c] = cy + T + kHalo; ye[c] = 1; }
-            else if (cy >= T - kHalo) { nty[c] = (ty[c] + 1) % NT; b1[c] = cy - T + kHalo; ye[c] = 1; }
+        for (int q = 0; q < QB; ++q) {
+            const int gq = threadIdx.x + 256 * (ib + q);
+            const int x = 4 * (gq % QPR), y = y0 + gq / QPR;
+            tx[q] = x / T;
+            ty[q] = y / T;
+            const int cx = x - tx[q] * T, cy = y - ty[q] * T;
+            a0[q] = cx + kHalo;
+            b0[q] = b1[q] = cy + kHalo;
+            ntx[q] = tx[q];
+            nty[q] = ty[q];
+            a1[q] = T + kHalo;          // a quad away from the x edges reads (and drops) this
+            xe[q] = ye[q] = 0;
+            if (cx == 0) { ntx[q] = (tx[q] + NT - 1) % NT; a1[q] = T + kHalo; xe[q] = 1; }
+            else if (cx == T - 4) { ntx[q] = (tx[q] + 1) % NT; a1[q] = 0; xe[q] = 2; }
+            if (cy < kHalo) { nty[q] = (ty[q] + NT - 1) % NT; b1[q] = cy + T + kHalo; ye[q] = 1; }
+            else if (cy >= T - kHalo) { nty[q] = (ty[q] + 1) % NT; b1[q] = cy - T + kHalo; ye[q] = 1; }
         }
-        // options (0, 0) and (1, 0), every cell: s = t00 (+ t10 at an x edge)
+        // quad q's options (0, oy) and (1, oy) into s (oy = 0 assigns, oy = 1 adds)
+        auto options = [&](int q, int oy, const float4 (&h)[2], const float4 (&hn)[2],
+                           const float2 (&e)[2], const float2 (&en)[2]) {
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                const float4 t0 = ZN ? add4(h[g], hn[g]) : h[g];
+                const float2 t1 = ZN ? add2(e[g], en[g]) : e[g];
+                float4 v = oy ? add4(s[q][g], t0) : t0;
+                if (xe[q] == 1) { v.x += t1.x; v.y += t1.y; }
+                else if (xe[q] == 2) { v.z += t1.x; v.w += t1.y; }
+                s[q][g] = v;
+            }
+        };
         {
-            float v[kCellBatch][2][2], w[kCellBatch][2][2];
+            float4 h[QB][2], hn[QB][2];
+            float2 e[QB][2], en[QB][2];
 #pragma unroll
-            for (int c = 0; c < kCellBatch; ++c) {
-                load2(has_zn, tx[c], a0[c], ty[c], b0[c], v[c][0], w[c][0]);
-                load2(has_zn, ntx[c], a1[c], ty[c], b0[c], v[c][1], w[c][1]);
+            for (int q = 0; q < QB; ++q) {
+                ld4(at(zb, ty[q], tx[q], a0[q], b0[q], czh), h[q]);
+                ld2(at(zb, ty[q], ntx[q], a1[q], b0[q], czh), e[q]);
+                if constexpr (ZN) {
+                    ld4(at(znb, ty[q], tx[q], a0[q], b0[q], czn), hn[q]);
+                    ld2(at(znb, ty[q], ntx[q], a1[q], b0[q], czn), en[q]);
+                }
             }
 #pragma unroll
-            for (int c = 0; c < kCellBatch; ++c) {
-                float t[2][2];
+            for (int q = 0; q < QB; ++q) options(q, 0, h[q], hn[q], e[q], en[q]);
+        }
 #pragma unroll
-                for (int o = 0; o < 2; ++o)
-#pragma unroll
-                    for (int g = 0; g < 2; ++g) t[o][g] = ZN ? v[c][o][g] + w[c][o][g] : v[c][o][g];
-                s0[c] = xe[c] ? t[0][0] + t[1][0] : t[0][0];
-                s1[c] = xe[c] ? t[0][1] + t[1][1] : t[0][1];
+        for (int q = 0; q < QB; ++q) {
+            if (!ye[q]) continue;
+            float4 h[2], hn[2];
+            float2 e[2], en[2];
+            ld4(at(zb, nty[q], tx[q], a0[q], b1[q], czh), h);
+            ld2(at(zb, nty[q], ntx[q], a1[q], b1[q], czh), e);
+            if constexpr (ZN) {
+                ld4(at(znb, nty[q], tx[q], a0[q], b1[q], czn), hn);
+                ld2(at(znb, nty[q], ntx[q], a1[q], b1[q], czn), en);
             }
-        }
-        // options (0, 1) and (1, 1), rows at a y edge (whole waves): s += t01 (+ t11)
-#pragma unroll
-        for (int c = 0; c < kCellBatch; ++c) {
-            if (!ye[c]) continue;
-            float v[2][2], w[2][2];
-            load2(has_zn, tx[c], a0[c], nty[c], b1[c], v[0], w[0]);
-            load2(has_zn, ntx[c], a1[c], nty[c], b1[c], v[1], w[1]);
-            float t[2][2];
-#pragma unroll
-            for (int o = 0; o < 2; ++o)
-#pragma unroll
-                for (int g = 0; g < 2; ++g) t[o][g] = ZN ? v[o][g] + w[o][g] : v[o][g];
-            s0[c] += t[0][0];
-            s1[c] += t[0][1];
-            if (xe[c]) { s0[c] += t[1][0]; s1[c] += t[1][1]; }
+            options(q, 1, h, hn, e, en);
         }
 #pragma unroll
-        for (int c = 0; c < kCellBatch; ++c) {
-            const int e = threadIdx.x + 256 * (ib + c);
-            const int x = e % N, yr = e / N;
-            rows[yr * RS + (x / B) * PB + x % B] = make_float2(s0[c], -s1[c]);
+        for (int q = 0; q < QB; ++q) {
+            const int gq = threadIdx.x + 256 * (ib + q);
+            const int x = 4 * (gq % QPR), yr = gq / QPR;
+            const float re[4] = {s[q][0].x, s[q][0].y, s[q][0].z, s[q][0].w};
+            const float im[4] = {s[q][1].x, s[q][1].y, s[q][1].z, s[q][1].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                rows[yr * RS + ((x + j) / B) * PB + (x + j) % B] = make_float2(re[j], -im[j]);
         }
     };
-    for (int ib = 0; ib < CPT; ib += kCellBatch) {
+    for (int ib = 0; ib < QPT; ib += QB) {
         if (zn) sum_batch(std::true_type{}, ib);
         else sum_batch(std::false_type{}, ib);
     }
@@ -1137,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_gather_xfft(const float *regions, const
     __syncthreads();
     float2 *o = out + ((int64_t)z * N + y0) * N;
 #pragma unroll
-    for (int i = 0; i < CPT; ++i) {
+    for (int i = 0; i < N * kXfftRows / 256; ++i) {
         const int e = threadIdx.x + 256 * i;
         const int k = e % N, yr = e / N;
         const float2 f = rows[yr * RS + (k % A) * PB + k / A];
@@ -1308,7 +1326,9 @@ void launch_gather_xfft(const AssignPlan &p, const float *regions, const float2 
     PBI4_CHECK(gather_xfft_supported(p), PBI4_EUNSUPPORTED,
                "fused gather + x transform: interlaced grids 128, 192, 256, 384 with tile edge 8, "
                "12 or 16 only");
-    const dim3 grid(p.N / kXfftRows, p.N);
+    PBI4_CHECK((reinterpret_cast<uintptr_t>(regions) & 15) == 0, PBI4_EINVAL,
+               "tile regions must be 16-byte aligned");
+    const dim3 grid(p.N / xfft_rows(p.N), p.N);
     auto run = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, regions, tw, out); };
 #define PBI4_GX(N, A, B) \
     (p.T == 8 ? run(k_gather_xfft<N, A, B, 8>) : p.T == 16 ? run(k_gather_xfft<N, A, B, 16>) \

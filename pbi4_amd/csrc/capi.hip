@@ -982,7 +982,10 @@ int pbi4_assign_density(pbi4_ctx *ctx, const pbi4_assign_desc *ad, const pbi4_de
                        ad->interlacing == dd->interlacing,
                    PBI4_EINVAL, "assignment and density descriptors differ in grid/order/interlacing");
         AssignPlan p = make_assign_plan(*ad, npart);
-        fused = gather_xfft_supported(p) && !std::getenv("PBI4_DENSITY_FFT");
+        // the fused gather reads the regions in 16-byte vectors: a workspace off that alignment
+        // takes the two-step route
+        fused = gather_xfft_supported(p) && !std::getenv("PBI4_DENSITY_FFT") &&
+                (reinterpret_cast<uintptr_t>(ws) & 15) == 0;
         if (!fused) return;
         hipStream_t s = (hipStream_t)stream;
         Arena a(ws, ws_bytes);
