@@ -871,7 +871,19 @@ __global__ __launch_bounds__(32 * XB) void k_zfft_combine(const float2 *ph, cons
     static_assert(A * B == N, "factors of N");
     constexpr int NT = 32 * XB, NC = 4 * XB, PB = B + 1, CS = A * PB + 1, H = N / 2 + 1;
     __shared__ float2 col[NC * CS];
-    const int x0 = blockIdx.x * XB, iy = blockIdx.y;
+    // work-groups are dealt round-robin to the 8 XCDs: hardware order h runs logical block L,
+    // consecutive L on one XCD, so the two blocks whose 64-byte runs share a 128-byte line
+    // (x0 = 16 k and 16 k + 8, on both sides) read it through one L2
+    int bx, iy;
+    {
+        const int total = gridDim.x * gridDim.y;
+        const int h = blockIdx.x + gridDim.x * blockIdx.y;
+        const int q = total / 8, r = total % 8, xcd = h % 8;
+        const int L = xcd * q + min(xcd, r) + h / 8;
+        iy = L / gridDim.x;
+        bx = L - iy * gridDim.x;
+    }
+    const int x0 = bx * XB;
     const int icy = (N - iy) % N;
     // column c = xi + XB (side + 2 ysel): X = side ? N - ix : ix (mod N), Y = ysel ? icy : iy;
     // ix past N / 2 (the last block) loads the cells of ix mod N and writes nothing
