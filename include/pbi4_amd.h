@@ -89,7 +89,7 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
 
 /* pbi4_assign followed by pbi4_compute_density in one call: the two steps the reference runs
  * back to back (pbi4/pbi4.py:304-341, DENSITIES.assign_* then compute_density).  On interlaced
- * grids 128, 192, 256 and 384 the gather of the tile regions is fused with the FFT's x axis and
+ * grids 128, 192, 256, 384 and 512 the gather of the tile regions is fused with the FFT's x axis and
  * the FFT runs in grids (no rocFFT pass); elsewhere, or with PBI4_DENSITY_FFT set, it is exactly
  * pbi4_assign + pbi4_compute_density.  grids (2 N^3 floats, or N^3 single grid) is scratch here
  * and holds no defined values afterwards; ws must hold max(pbi4_assign_workspace,

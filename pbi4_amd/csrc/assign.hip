@@ -1315,7 +1315,7 @@ const float *assign_run(const AssignPlan &p, const float *pos, const float *vel,
 bool gather_xfft_supported(const AssignPlan &p) {
     if (p.ngrids != 2 || p.R != p.T + 2 * kHalo) return false;
     switch (p.N) {
-        case 128: case 256: return p.T == 8 || p.T == 16;
+        case 128: case 256: case 512: return p.T == 8 || p.T == 16;
         case 192: case 384: return p.T == 8 || p.T == 12 || p.T == 16;
         default: return false;
     }
@@ -1324,8 +1324,8 @@ bool gather_xfft_supported(const AssignPlan &p) {
 void launch_gather_xfft(const AssignPlan &p, const float *regions, const float2 *tw, float2 *out,
                         hipStream_t s) {
     PBI4_CHECK(gather_xfft_supported(p), PBI4_EUNSUPPORTED,
-               "fused gather + x transform: interlaced grids 128, 192, 256, 384 with tile edge 8, "
-               "12 or 16 only");
+               "fused gather + x transform: interlaced grids 128, 192, 256, 384, 512 with tile "
+               "edge 8, 12 or 16 only");
     PBI4_CHECK((reinterpret_cast<uintptr_t>(regions) & 15) == 0, PBI4_EINVAL,
                "tile regions must be 16-byte aligned");
     const dim3 grid(p.N / xfft_rows(p.N), p.N);
@@ -1337,7 +1337,8 @@ void launch_gather_xfft(const AssignPlan &p, const float *regions, const float2 
         case 128: PBI4_GX(128, 16, 8); break;
         case 192: PBI4_GX(192, 16, 12); break;
         case 256: PBI4_GX(256, 16, 16); break;
-        default: PBI4_GX(384, 16, 24); break;
+        case 384: PBI4_GX(384, 16, 24); break;
+        default: PBI4_GX(512, 16, 32); break;
     }
 #undef PBI4_GX
     PBI4_HIP(hipGetLastError());

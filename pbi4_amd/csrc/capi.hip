@@ -680,7 +680,7 @@ struct DensityWs {
 };
 // Interlaced: one complex FFT of grid1 + i grid2 (the array the reference transforms,
 // pbi4.py:337), the two grids passed as its planar real and imaginary parts; on the grids 128,
-// 192, 256, 384 rocFFT transforms the z planes over (y, x) and the combine kernel does the z axis
+// 192, 256, 384, 512 rocFFT transforms the z planes over (y, x) and the combine kernel does the z axis
 // (PBI4_DENSITY_FFT=rocfft3d keeps rocFFT's 3D transform and the separate combine); single
 // grid: r2c.
 bool density_fused_z(const pbi4_density_desc &d) {

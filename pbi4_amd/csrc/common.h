@@ -71,7 +71,7 @@ size_t assign_workspace(const AssignPlan &p, int64_t npart);
 // ws for launch_gather_xfft.
 const float *assign_run(const AssignPlan &p, const float *pos, const float *vel, int64_t npart,
                         float *grids, Arena &ws, hipStream_t s, KernelTimer *timer);
-// Interlaced grids 128, 192, 256, 384: the gather fused with the density FFT's x axis, out[z][y][k]
+// Interlaced grids 128, 192, 256, 384, 512: the gather fused with the density FFT's x axis, out[z][y][k]
 // = forward DFT over x of grid1 + i grid2 (cells bitwise the gathered ones); tw[k] = e^{2 pi i k / N}.
 bool gather_xfft_supported(const AssignPlan &p);
 void launch_gather_xfft(const AssignPlan &p, const float *regions, const float2 *tw, float2 *out,
@@ -84,12 +84,12 @@ void launch_combine(int N, int interlacing, const float *wtab, const float2 *ph,
                     const float2 *G1, float2 *delta, hipStream_t s);
 
 // Interlaced density with the FFT's z axis fused into the combine: G holds every z plane of
-// grid1 + i grid2 transformed over (y, x); tw[k] = e^{2 pi i k / N}.  Grids 128, 192, 256, 384.
+// grid1 + i grid2 transformed over (y, x); tw[k] = e^{2 pi i k / N}.  Grids 128, 192, 256, 384, 512.
 bool zfft_combine_supported(int N);
 void launch_zfft_combine(int N, const float *wtab, const float2 *ph, const float2 *tw, float cf,
                          const float2 *G, float2 *delta, hipStream_t s);
 // In place, every (z, x) column of G[z][y][x] (N^3 complex) replaced by its forward DFT over y;
-// grids 128, 192, 256, 384.
+// grids 128, 192, 256, 384, 512.
 void launch_yfft(int N, const float2 *tw, float2 *G, hipStream_t s);
 
 void launch_fcomb_full(int N, int interlacing, const float *wtab, float cf, const float2 *F,

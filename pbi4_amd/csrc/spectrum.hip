@@ -1005,13 +1005,14 @@ __global__ __launch_bounds__(256) void k_yfft(const float2 *tw, float2 *G) {
 }  // namespace
 
 void launch_yfft(int N, const float2 *tw, float2 *G, hipStream_t s) {
-    PBI4_CHECK(zfft_combine_supported(N), PBI4_EUNSUPPORTED, "y transform: grids 128, 192, 256, 384");
+    PBI4_CHECK(zfft_combine_supported(N), PBI4_EUNSUPPORTED, "y transform: grids 128, 192, 256, 384, 512");
     const dim3 grid(N / 16, N);
     switch (N) {
         case 128: hipLaunchKernelGGL((k_yfft<128, 16, 8>), grid, dim3(256), 0, s, tw, G); break;
         case 192: hipLaunchKernelGGL((k_yfft<192, 16, 12>), grid, dim3(256), 0, s, tw, G); break;
         case 256: hipLaunchKernelGGL((k_yfft<256, 16, 16>), grid, dim3(256), 0, s, tw, G); break;
-        default: hipLaunchKernelGGL((k_yfft<384, 16, 24>), grid, dim3(256), 0, s, tw, G); break;
+        case 384: hipLaunchKernelGGL((k_yfft<384, 16, 24>), grid, dim3(256), 0, s, tw, G); break;
+        default: hipLaunchKernelGGL((k_yfft<512, 16, 32>), grid, dim3(256), 0, s, tw, G); break;
     }
     PBI4_HIP(hipGetLastError());
 }
@@ -1109,7 +1110,7 @@ void launch_combine(int N, int interlacing, const float *wtab, const float2 *ph,
     PBI4_HIP(hipGetLastError());
 }
 
-bool zfft_combine_supported(int N) { return N == 128 || N == 192 || N == 256 || N == 384; }
+bool zfft_combine_supported(int N) { return N == 128 || N == 192 || N == 256 || N == 384 || N == 512; }
 
 void launch_zfft_combine(int N, const float *wtab, const float2 *ph, const float2 *tw, float cf,
                          const float2 *G, float2 *delta, hipStream_t s) {
@@ -1127,6 +1128,8 @@ void launch_zfft_combine(int N, const float *wtab, const float2 *ph, const float
         case 192: run(k_zfft_combine<192, 16, 12, 8>, 8); break;
         case 256: run(k_zfft_combine<256, 16, 16, 8>, 8); break;
         case 384: run(k_zfft_combine<384, 16, 24, 8>, 8); break;
+        // 4 x-columns per side: 16 columns of 512 entries are 68 KB of LDS
+        case 512: run(k_zfft_combine<512, 16, 32, 4>, 4); break;
         default: fail(PBI4_EUNSUPPORTED, "no fused z transform for this grid");
     }
     PBI4_HIP(hipGetLastError());

@@ -1,6 +1,6 @@
 """Interlaced density with the FFT's z axis fused into the combine, against rocFFT's 3D transform.
 
-On the grids 128, 192, 256 and 384 pbi4_compute_density has rocFFT transform every z plane of
+On the grids 128, 192, 256, 384 and 512 pbi4_compute_density has rocFFT transform every z plane of
 grid1 + i grid2 over (y, x) and one kernel does the forward DFT along z in LDS and the
 interlacing combine (k_zfft_combine); PBI4_DENSITY_FFT=rocfft3d keeps rocFFT's 3D c2c and the
 separate combine.  Both are unnormalised FP32 transforms of the same grids followed by the same
@@ -43,7 +43,7 @@ def _density(ctx, g, N, n, mode):
             os.environ["PBI4_DENSITY_FFT"] = old
 
 
-@pytest.mark.parametrize("N", [128, 192, 256, 384])
+@pytest.mark.parametrize("N", [128, 192, 256, 384, 512])
 def test_fused_z_transform_matches_rocfft_3d(ctx, N):
     from pbi4_amd.catalogue import zeldovich
     print(f"fused z transform + combine vs rocFFT 3D: N={N}", flush=True)
@@ -76,7 +76,7 @@ def _two_step(ctx, p, N, order, interlacing, n, vel=None, rsd=None):
 
 @pytest.mark.parametrize("N,npart_side,order,rsd", [
     (128, 128, 4, False), (128, 192, 3, True), (192, 256, 4, True), (256, 256, 4, True),
-    (384, 192, 2, False)])
+    (384, 192, 2, False), (512, 256, 4, True)])
 def test_assign_density_fused_gather_matches_two_steps(ctx, N, npart_side, order, rsd):
     """pbi4_assign_density on the fused route (gather + x transform, y transform, z transform +
     combine) against pbi4_assign + pbi4_compute_density (gather, rocFFT over (y, x), z transform +
