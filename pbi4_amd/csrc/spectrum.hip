@@ -863,13 +863,14 @@ __global__ __launch_bounds__(256) void k_maps_rows(int C, int ld, int64_t nrows,
 // four-step transform of the conjugate, conjugated again when read), then evaluates the octets
 // as k_combine_interlaced does.  In a column, z = j + B m sits at m (B + 1) + j and, after the two
 // steps, F[k1 + A k2] at k1 (B + 1) + k2: the second step's lanes (one k1 each) read B + 1
-// entries apart, on distinct LDS banks.  Threads: 4 XB x 8 z rows.
-template <int N, int A, int B, int XB>
-__global__ __launch_bounds__(32 * XB) void k_zfft_combine(const float2 *ph, const float *wtab,
+// entries apart, on distinct LDS banks.  Threads: 4 XB columns x NT / (4 XB) z rows.
+template <int N, int A, int B, int XB, int NT>
+__global__ __launch_bounds__(NT) void k_zfft_combine(const float2 *ph, const float *wtab,
                                                           float cf, const float2 *tw,
                                                           const float2 *G, float2 *delta) {
     static_assert(A * B == N, "factors of N");
-    constexpr int NT = 32 * XB, NC = 4 * XB, PB = B + 1, CS = A * PB + 1, H = N / 2 + 1;
+    static_assert(NT % (4 * XB) == 0 && N % (NT / (4 * XB)) == 0, "threads per column set");
+    constexpr int NC = 4 * XB, PB = B + 1, CS = A * PB + 1, H = N / 2 + 1;
     __shared__ float2 col[NC * CS];
     // work-groups are dealt round-robin to the 8 XCDs: hardware order h runs logical block L,
     // consecutive L on one XCD, so the two blocks whose 64-byte runs share a 128-byte line
@@ -1115,21 +1116,23 @@ bool zfft_combine_supported(int N) { return N == 128 || N == 192 || N == 256 || 
 void launch_zfft_combine(int N, const float *wtab, const float2 *ph, const float2 *tw, float cf,
                          const float2 *G, float2 *delta, hipStream_t s) {
     PBI4_CHECK(ph != nullptr && tw != nullptr, PBI4_EINVAL, "the fused z transform needs its tables");
-    auto run = [&](auto kern, int xb) {
+    auto run = [&](auto kern, int xb, int nt) {
         const dim3 grid((unsigned)ceil_div(N / 2 + 1, xb), (unsigned)(N / 2 + 1));
-        hipLaunchKernelGGL(kern, grid, dim3(32 * xb), 0, s, ph, wtab, cf, tw, G, delta);
+        hipLaunchKernelGGL(kern, grid, dim3(nt), 0, s, ph, wtab, cf, tw, G, delta);
     };
     // 8 x-columns per side (64-byte runs, 66 KB of LDS at 256^3, two work-groups per CU): 16
     // (128-byte runs, 140 KB, one work-group per CU) took 78.5 against 90.5 us alone at 256^3 but
     // the six-slot serving stream ran slower with it (435 against 451 catalogues/s), its LDS
-    // keeping other slots' work-groups off the CU
+    // keeping other slots' work-groups off the CU.  512 threads (256 on 512) rather than 256
+    // (128): the LDS caps the work-groups per CU, so more threads per work-group keep more
+    // loads in flight (256^3: 61.3 -> 59.0 us; 512^3: 848 -> 708 us)
     switch (N) {
-        case 128: run(k_zfft_combine<128, 16, 8, 8>, 8); break;
-        case 192: run(k_zfft_combine<192, 16, 12, 8>, 8); break;
-        case 256: run(k_zfft_combine<256, 16, 16, 8>, 8); break;
-        case 384: run(k_zfft_combine<384, 16, 24, 8>, 8); break;
+        case 128: run(k_zfft_combine<128, 16, 8, 8, 512>, 8, 512); break;
+        case 192: run(k_zfft_combine<192, 16, 12, 8, 512>, 8, 512); break;
+        case 256: run(k_zfft_combine<256, 16, 16, 8, 512>, 8, 512); break;
+        case 384: run(k_zfft_combine<384, 16, 24, 8, 512>, 8, 512); break;
         // 4 x-columns per side: 16 columns of 512 entries are 68 KB of LDS
-        case 512: run(k_zfft_combine<512, 16, 32, 4>, 4); break;
+        case 512: run(k_zfft_combine<512, 16, 32, 4, 256>, 4, 256); break;
         default: fail(PBI4_EUNSUPPORTED, "no fused z transform for this grid");
     }
     PBI4_HIP(hipGetLastError());
