@@ -93,7 +93,7 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
  * the FFT runs in grids (no rocFFT pass); elsewhere, or with PBI4_DENSITY_FFT set, it is exactly
  * pbi4_assign + pbi4_compute_density.  grids (2 N^3 floats, or N^3 single grid) is scratch here
  * and holds no defined values afterwards; ws must hold max(pbi4_assign_workspace,
- * pbi4_density_workspace) bytes.  ad and dd must name the same grid, order and interlacing.
+ * pbi4_density_workspace) bytes (a ws that is not 16-byte aligned takes the two-step route).  ad and dd must name the same grid, order and interlacing.
  */
 int pbi4_assign_density(pbi4_ctx *ctx, const pbi4_assign_desc *ad, const pbi4_density_desc *dd,
                         const float *pos, const float *vel, int64_t npart, float *grids,

@@ -7,7 +7,9 @@
 // LDS atomics.  Each tile writes its halo-extended block to a region buffer in pieces grouped by
 // the tile that owns the cells; a gather pass then sums, for every grid cell, the (1 to 8)
 // region entries that cover it in a fixed order.  No global atomics touch grid memory, and the
-// output is bitwise reproducible.
+// output is bitwise reproducible.  When the density follows in the same call, the gather is fused
+// with the first axis of the density's FFT (k_gather_xfft: the same cells, transformed along x
+// before they leave LDS).
 //
 // Stencils relative to the cell c = floor(u) containing the particle (u = N * pos), written
 // with the fractional part f = u - c, follow the reference exactly:
