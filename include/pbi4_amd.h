@@ -89,11 +89,12 @@ int pbi4_compute_density(pbi4_ctx *ctx, const pbi4_density_desc *d, const float 
 
 /* pbi4_assign followed by pbi4_compute_density in one call: the two steps the reference runs
  * back to back (pbi4/pbi4.py:304-341, DENSITIES.assign_* then compute_density).  On interlaced
- * grids 128, 192, 256, 384 and 512 the gather of the tile regions is fused with the FFT's x axis and
- * the FFT runs in grids (no rocFFT pass); elsewhere, or with PBI4_DENSITY_FFT set, it is exactly
- * pbi4_assign + pbi4_compute_density.  grids (2 N^3 floats, or N^3 single grid) is scratch here
- * and holds no defined values afterwards; ws must hold max(pbi4_assign_workspace,
- * pbi4_density_workspace) bytes (a ws that is not 16-byte aligned takes the two-step route).  ad and dd must name the same grid, order and interlacing.
+ * grids 128, 192, 256, 384 and 512 the gather of the tile regions is fused with the FFT's x axis
+ * and the FFT runs in grids (no rocFFT pass); elsewhere, or with PBI4_DENSITY_FFT set, it is
+ * exactly pbi4_assign + pbi4_compute_density.  grids (2 N^3 floats, or N^3 single grid) is
+ * scratch here and holds no defined values afterwards; ws must hold max(pbi4_assign_workspace,
+ * pbi4_density_workspace) bytes (a ws that is not 16-byte aligned takes the two-step route).
+ * ad and dd must name the same grid, order and interlacing.
  */
 int pbi4_assign_density(pbi4_ctx *ctx, const pbi4_assign_desc *ad, const pbi4_density_desc *dd,
                         const float *pos, const float *vel, int64_t npart, float *grids,
