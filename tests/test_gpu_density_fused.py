@@ -127,3 +127,26 @@ def test_assign_density_other_shapes_are_the_two_steps(ctx, N, interlacing):
     _, ref = _two_step(ctx, p, N, 3, interlacing, n)
     got = ctx.assign_density(p, N, 3, interlacing, 1000.0, n, normalised=False)
     assert torch.equal(torch.view_as_real(got), torch.view_as_real(ref))
+
+
+def test_assign_density_rejects_mismatched_descriptors(ctx):
+    """The assignment and density descriptors must name the same grid, order and interlacing:
+    a mismatch is refused before any launch, with the reason in pbi4_last_error."""
+    import ctypes
+    from pbi4_amd import _lib
+    from pbi4_amd._lib import AssignDesc, DensityDesc, lib
+    print("assign_density: mismatched descriptors", flush=True)
+    n = 1000
+    pos = torch.rand((3, n), dtype=torch.float32, device="cuda")
+    ad = AssignDesc(128, 4, 1, _lib.POS_NORMALISED, 1.0, (ctypes.c_float * 3)(0.0, 0.0, 0.0))
+    dd = DensityDesc(256, 4, 1, 1000.0, float(n))
+    grids = torch.empty((2, 256, 256, 256), dtype=torch.float32, device="cuda")
+    delta = torch.empty((256, 256, 129), dtype=torch.complex64, device="cuda")
+    ws = torch.empty(1 << 20, dtype=torch.uint8, device="cuda")
+    rc = lib().pbi4_assign_density(ctx.h, ctypes.byref(ad), ctypes.byref(dd),
+                                   ctypes.c_void_p(pos.data_ptr()), None, n,
+                                   ctypes.c_void_p(grids.data_ptr()),
+                                   ctypes.c_void_p(delta.data_ptr()),
+                                   ctypes.c_void_p(ws.data_ptr()), ws.numel(), None)
+    assert rc == -1                                     # PBI4_EINVAL
+    assert b"differ" in lib().pbi4_last_error(ctx.h)
