@@ -951,11 +951,12 @@ __global__ __launch_bounds__(NT) void k_zfft_combine(const float2 *ph, const flo
 // The middle axis of the interlaced density's FFT, in place: a work-group loads the 16 y-columns
 // (z, x0 .. x0 + 15) of G[z][y][x] (128-byte runs), transforms each over y in LDS (forward DFT of
 // length N = A B as the inverse four-step transform of the conjugate; same layout as
-// k_zfft_combine) and writes them back.  Threads: 16 x by 16 y rows.
-template <int N, int A, int B>
-__global__ __launch_bounds__(256) void k_yfft(const float2 *tw, float2 *G) {
+// k_zfft_combine) and writes them back.  Threads: 16 x by NT / 16 y rows.
+template <int N, int A, int B, int NT>
+__global__ __launch_bounds__(NT) void k_yfft(const float2 *tw, float2 *G) {
     static_assert(A * B == N && N % 16 == 0, "factors of N");
-    constexpr int XB = 16, PB = B + 1, CS = A * PB + 1, NY = N / 16;
+    constexpr int XB = 16, RW = NT / XB, PB = B + 1, CS = A * PB + 1, NY = N / RW;
+    static_assert(N % RW == 0, "rows per thread");
     __shared__ float2 col[XB * CS];
     const int x0 = blockIdx.x * XB, z = blockIdx.y;
     const int xi = threadIdx.x % XB, yr = threadIdx.x / XB;
@@ -963,16 +964,16 @@ __global__ __launch_bounds__(256) void k_yfft(const float2 *tw, float2 *G) {
     {
         float2 v[NY];
 #pragma unroll
-        for (int i = 0; i < NY; ++i) v[i] = g[(int64_t)(yr + 16 * i) * N];
+        for (int i = 0; i < NY; ++i) v[i] = g[(int64_t)(yr + RW * i) * N];
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
-            const int y = yr + 16 * i;
+            const int y = yr + RW * i;
             col[xi * CS + (y / B) * PB + y % B] = make_float2(v[i].x, -v[i].y);
         }
     }
     __syncthreads();
     // step 1: per column and j < B, the A entries y = j + B m over m, times W_N^{j k1}, in place
-    for (int t = threadIdx.x; t < XB * B; t += 256) {
+    for (int t = threadIdx.x; t < XB * B; t += NT) {
         const int c = t / B, j = t - c * B;
         float2 *cc = col + c * CS + j;
         float2 x[A];
@@ -984,7 +985,7 @@ __global__ __launch_bounds__(256) void k_yfft(const float2 *tw, float2 *G) {
     }
     __syncthreads();
     // step 2: per column and k1 < A, the B entries k1 (B + 1) + j over j, in place
-    for (int t = threadIdx.x; t < XB * A; t += 256) {
+    for (int t = threadIdx.x; t < XB * A; t += NT) {
         const int c = t / A, k1 = t - c * A;
         float2 *cc = col + c * CS + k1 * PB;
         float2 x[B];
@@ -997,7 +998,7 @@ __global__ __launch_bounds__(256) void k_yfft(const float2 *tw, float2 *G) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NY; ++i) {
-        const int y = yr + 16 * i;
+        const int y = yr + RW * i;
         const float2 f = col[xi * CS + (y % A) * PB + y / A];
         g[(int64_t)y * N] = make_float2(f.x, -f.y);
     }
@@ -1009,11 +1010,13 @@ void launch_yfft(int N, const float2 *tw, float2 *G, hipStream_t s) {
     PBI4_CHECK(zfft_combine_supported(N), PBI4_EUNSUPPORTED, "y transform: grids 128, 192, 256, 384, 512");
     const dim3 grid(N / 16, N);
     switch (N) {
-        case 128: hipLaunchKernelGGL((k_yfft<128, 16, 8>), grid, dim3(256), 0, s, tw, G); break;
-        case 192: hipLaunchKernelGGL((k_yfft<192, 16, 12>), grid, dim3(256), 0, s, tw, G); break;
-        case 256: hipLaunchKernelGGL((k_yfft<256, 16, 16>), grid, dim3(256), 0, s, tw, G); break;
-        case 384: hipLaunchKernelGGL((k_yfft<384, 16, 24>), grid, dim3(256), 0, s, tw, G); break;
-        default: hipLaunchKernelGGL((k_yfft<512, 16, 32>), grid, dim3(256), 0, s, tw, G); break;
+        // 256 threads: with 512 the LDS still holds 4 work-groups per CU and 256^3 took 52.8
+        // against 52.1 us
+        case 128: hipLaunchKernelGGL((k_yfft<128, 16, 8, 256>), grid, dim3(256), 0, s, tw, G); break;
+        case 192: hipLaunchKernelGGL((k_yfft<192, 16, 12, 256>), grid, dim3(256), 0, s, tw, G); break;
+        case 256: hipLaunchKernelGGL((k_yfft<256, 16, 16, 256>), grid, dim3(256), 0, s, tw, G); break;
+        case 384: hipLaunchKernelGGL((k_yfft<384, 16, 24, 256>), grid, dim3(256), 0, s, tw, G); break;
+        default: hipLaunchKernelGGL((k_yfft<512, 16, 32, 256>), grid, dim3(256), 0, s, tw, G); break;
     }
     PBI4_HIP(hipGetLastError());
 }
